@@ -1,0 +1,210 @@
+/*
+ * shinobi_grid.h -- C ABI of the MI355X (gfx950) grid-acceleration hot path.
+ *
+ * The reference wires its builder, primitive storage, intersector and
+ * traverser together as compile-time C++ concepts
+ * (src/Engine/RayTracingTypes.h:84-128, src/Shinobi.h:48-120); there is no
+ * FFI.  This header is the C-ABI replacement of that wiring: plain pointers,
+ * sizes and status codes, no HIP or torch types in the signatures (streams
+ * are passed as void* and are hipStream_t values; NULL = the null stream).
+ * include/shinobi/grid.hpp restores the reference's C++ concepts on top.
+ *
+ * Threading: calls on different sg_ctx / sg_grid objects may run
+ * concurrently; one object must not be used from two threads at once.
+ * Errors: every entry point returns an sg_status; nothing prints or exits
+ * (the reference's CUDA_SAFE_CALL / CUT_CHECK_ERROR print and exit,
+ * src/CUDAStdAfx.h:38).
+ */
+#ifndef SHINOBI_GRID_H
+#define SHINOBI_GRID_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int sg_status;
+enum {
+    SG_OK = 0,
+    SG_ERR_INVALID_ARG = 1,     /* null pointer, bad enum, size overflow          */
+    SG_ERR_INVALID_SCENE = 2,   /* empty scene, zero-extent bounds, res < 1       */
+    SG_ERR_OUT_OF_MEMORY = 3,   /* device allocation failed                        */
+    SG_ERR_HIP = 4,             /* a HIP runtime call failed                       */
+    SG_ERR_CAPACITY = 5,        /* a rebuild outgrew the grid's buffers            */
+    SG_ERR_LEAF_RES = 6,        /* a leaf resolution left [1,255] (reference D5)  */
+    SG_ERR_TOO_MANY_CELLS = 7,  /* >= 2^24 cells: float key products inexact      */
+    SG_ERR_NEG_EXTENT = 8,      /* a negative leaf extent (reference corrupts)     */
+    SG_ERR_TIMEOUT = 9,         /* a device-side wait exceeded its bound           */
+    SG_ERR_BUDGET = 10,         /* request exceeds the serving HBM budget          */
+    SG_ERR_NO_DEVICE = 11       /* no HIP device                                   */
+};
+
+const char *sg_status_str(sg_status s);
+
+typedef struct sg_ctx sg_ctx;
+typedef struct sg_scene sg_scene;
+typedef struct sg_grid sg_grid;
+
+enum { SG_GRID_UNIFORM = 0, SG_GRID_TWO_LEVEL = 1 };
+
+/* Build parameters.  Defaults (sg_build_params_default) are the reference's:
+ * lambda 5 (GridBuilder.h:48, FastTwoLevelGridBuilder.h:90), top divisor 6
+ * (FastTwoLevelGridBuilder.h:100-102), leaf lambda 1.2 and the 16-reference
+ * leaf threshold (TwoLevelBuildKernels.h:49,73). */
+typedef struct {
+    int kind;                 /* SG_GRID_UNIFORM or SG_GRID_TWO_LEVEL          */
+    float lambda;             /* top-level (or uniform) density                */
+    int top_divisor;          /* two-level top resolution divisor              */
+    float lambda_leaf;        /* leaf density                                  */
+    uint32_t leaf_min_refs;   /* <= this many refs: the top cell is one leaf   */
+    int res_override[3];      /* all > 0: use this resolution                  */
+    int radix_bits;           /* 4, 6 or 8 (0 = 8)                             */
+    float capacity_slack;     /* buffer headroom for rebuilds (0 = 1.25)       */
+    int keep_debug;           /* keep per-triangle / per-pair counts           */
+} sg_build_params;
+
+void sg_build_params_default(sg_build_params *p, int kind);
+
+typedef struct {
+    int kind;
+    sg_status status;
+    int res[3];
+    float bmin[3], bmax[3];
+    float cell_size[3];       /* (max-min)/res                                  */
+    float cell_size_rcp[3];   /* res/(max-min)  (traversal, TwoLevelGrid.h:84-87) */
+    float build_rcp[3];       /* 1/cell_size    (build, FastTwoLevelGridBuilder.h:123) */
+    uint32_t n_tris;
+    uint32_t C1, P1, P1_valid;  /* top cells, top pairs (incl. rejected), accepted */
+    uint32_t C2, P2, P2_valid;  /* leaf cells, leaf pairs, accepted leaf pairs     */
+    uint32_t cap_C1, cap_P1, cap_C2, cap_P2;
+    uint64_t device_bytes;
+} sg_grid_info;
+
+/* arrays retrievable with sg_grid_download (all uint32) */
+enum {
+    SG_ARRAY_TOP_KEYS = 0,     /* P1  sorted top-level pair keys               */
+    SG_ARRAY_TOP_VALS = 1,     /* P1  sorted top-level pair triangle ids       */
+    SG_ARRAY_TOP_RANGES = 2,   /* 2*C1 top-level [begin,end) (uniform: cells)  */
+    SG_ARRAY_TOP_CELLS = 3,    /* 2*C1 packed TwoLevelGridCell                 */
+    SG_ARRAY_LEAF_KEYS = 4,    /* P2                                           */
+    SG_ARRAY_LEAF_VALS = 5,    /* P2                                           */
+    SG_ARRAY_LEAVES = 6,       /* 2*C2 leaf [begin,end)                        */
+    SG_ARRAY_INDICES = 7,      /* P   reference -> triangle (FaceSoup::indices) */
+    SG_ARRAY_TRI_COUNTS = 8,   /* N   per-triangle top counts (keep_debug)     */
+    SG_ARRAY_LEAF_COUNTS = 9,  /* C1  leaf cells per top cell (keep_debug)     */
+    SG_ARRAY_PAIR_COUNTS = 10  /* P1  leaf refs per top pair (keep_debug)      */
+};
+
+/* rays: seven device arrays ox,oy,oz,dx,dy,dz,tmax (SoA, fp32) */
+typedef struct {
+    const float *o[3];
+    const float *d[3];
+    const float *tmax;
+} sg_rays;
+
+typedef struct {
+    float *o[3];
+    float *d[3];
+    float *tmax;
+} sg_rays_out;
+
+/* ------------------------------------------------------------ context --- */
+/* replaces the implicit single CUDA context of the reference */
+sg_status sg_ctx_create(int device, sg_ctx **out);
+sg_status sg_ctx_destroy(sg_ctx *ctx);
+sg_status sg_device_count(int *out);
+
+/* -------------------------------------------------------------- scene --- */
+/* Primitive storage: replaces FaceSoup::upload (src/Control/Structure/
+ * FaceSoup.h:171-247).  soa9 = v0.x,v0.y,v0.z,v1.x,...,v2.z, n floats each,
+ * in host memory (src_on_device = 0) or device memory (1).  The library keeps
+ * a device copy and a 48-byte traversal record per triangle. */
+sg_status sg_scene_create(sg_ctx *ctx, const float *const soa9[9], uint32_t n, int src_on_device,
+                          void *stream, sg_scene **out);
+/* new vertex positions, same triangle count: replaces FaceSoup::uploadVertices
+ * (src/Control/Structure/FaceSoup.h:249-339) */
+sg_status sg_scene_update(sg_scene *scene, const float *const soa9[9], int src_on_device, void *stream);
+sg_status sg_scene_destroy(sg_scene *scene);
+uint32_t sg_scene_size(const sg_scene *scene);
+/* device pointers of the scene's SoA arrays (9) */
+sg_status sg_scene_device_soa(const sg_scene *scene, const float *out9[9]);
+
+/* -------------------------------------------------------------- build --- */
+/* Builder::init + build (FastSimpleGridBuilder.h:59-327,
+ * FastTwoLevelGridBuilder.h:61-570; called by StaticRTEngine::build,
+ * src/Engine/StaticRTEngine.h:225-237).  Sizes the buffers (it may wait on
+ * the stream); the build itself is enqueued on `stream`. */
+sg_status sg_build(sg_ctx *ctx, sg_scene *scene, const sg_build_params *p, void *stream, sg_grid **out);
+/* Builder::rebuild (FastTwoLevelGridBuilder.h:573-740): same resolution,
+ * bounds and everything else recomputed on the device from the scene's
+ * current vertices, into the grid's existing buffers.  Never synchronizes,
+ * never allocates, captures into a hipGraph.  A rebuild that outgrows the
+ * buffers is reported by sg_grid_get_info as SG_ERR_CAPACITY. */
+sg_status sg_rebuild(sg_ctx *ctx, sg_scene *scene, sg_grid *grid, void *stream);
+/* like sg_rebuild but lets the device recompute the resolution as well */
+sg_status sg_rebuild_free_res(sg_ctx *ctx, sg_scene *scene, sg_grid *grid, void *stream);
+/* waits for the grid's stream, reads the device header */
+sg_status sg_grid_get_info(sg_grid *grid, sg_grid_info *out);
+sg_status sg_grid_download(sg_grid *grid, int which, void *host, size_t bytes);
+sg_status sg_grid_destroy(sg_grid *grid);
+
+/* ---------------------------------------------------------- traversal --- */
+/* The trace kernel contract (src/Engine/RayTracingKernels.h:37-136) with
+ * SimpleIntersector (src/Intersector/SimpleIntersector.hpp:312-337) and the
+ * grid's traverser (UniformGridTraverser / TwoLevelGridTraverser):
+ * rays in -> prim (triangle id, 0xFFFFFFFF on a miss) and final t.
+ * d_count: optional device pointer holding the ray count (<= n). */
+sg_status sg_trace_closest(sg_ctx *ctx, sg_grid *grid, const sg_rays *rays, uint32_t n,
+                           const uint32_t *d_count, uint32_t *d_prim, float *d_t, uint32_t *d_stats,
+                           void *stream);
+/* one cosine-weighted diffuse bounce per hit (SimplePathTracer.hpp:115-166),
+ * compacted in primary order; *d_count = number of bounce rays; d_src (opt.)
+ * = primary ray index of each bounce ray */
+sg_status sg_bounce(sg_ctx *ctx, sg_scene *scene, const sg_rays *rays, const uint32_t *d_prim,
+                    const float *d_t, uint32_t n, uint64_t seed, const sg_rays_out *out, uint32_t *d_src,
+                    uint32_t *d_count, void *stream);
+
+/* ---------------------------------------------------------- primitives --- */
+/* ExclusiveScan / InclusiveScan (src/Utils/Scan.h:35-68) on device arrays */
+sg_status sg_scan_u32(sg_ctx *ctx, const uint32_t *d_in, uint32_t *d_out, uint32_t n, int inclusive,
+                      void *stream);
+/* Sort (src/Utils/Sort.h:33-48): stable sort of n (key,value) pairs by the
+ * low key_bits bits of the key, in place on the device arrays */
+sg_status sg_sort_pairs(sg_ctx *ctx, uint32_t *d_keys, uint32_t *d_vals, uint32_t n, int key_bits,
+                        int radix_bits, void *stream);
+
+/* -------------------------------------------------- device plumbing --- */
+sg_status sg_malloc(sg_ctx *ctx, size_t bytes, void **out);
+sg_status sg_free(sg_ctx *ctx, void *p);
+sg_status sg_memcpy_htod(sg_ctx *ctx, void *dst, const void *src, size_t bytes, void *stream);
+sg_status sg_memcpy_dtoh(sg_ctx *ctx, void *dst, const void *src, size_t bytes, void *stream);
+sg_status sg_stream_create(sg_ctx *ctx, void **out);
+sg_status sg_stream_destroy(sg_ctx *ctx, void *stream);
+sg_status sg_stream_sync(sg_ctx *ctx, void *stream);
+sg_status sg_event_create(sg_ctx *ctx, void **out);
+sg_status sg_event_destroy(sg_ctx *ctx, void *ev);
+sg_status sg_event_record(sg_ctx *ctx, void *ev, void *stream);
+sg_status sg_event_elapsed_ms(sg_ctx *ctx, void *ev_start, void *ev_end, float *ms);
+
+/* ---------------------------------------------------------- generators --- */
+/* Seeded synthetic inputs (host memory), see shinobi_amd/csrc/sg_gen.cpp.
+ * scene kinds: 0 = uniform-in-cube soup (n triangles, edge scale e),
+ * 1 = height field (grid x grid vertices), 2 = dense cluster + large tris. */
+sg_status sg_gen_soup(uint64_t seed, uint32_t n, float edge, float *const soa9[9]);
+sg_status sg_gen_heightfield(uint64_t seed, uint32_t grid, float amplitude, float *const soa9[9]);
+float sg_heightfield_height(uint64_t seed, float x, float y, float amplitude);
+sg_status sg_gen_cluster(uint64_t seed, uint32_t n_small, uint32_t n_large, float *const soa9[9]);
+/* pinhole primary rays with the reference Camera (src/Primitive/Camera.h:41-70)
+ * and RegularPixelSampler (src/Primitive/PixelSampler.h:34-54), one per pixel
+ * centre; rays7 = 7 host arrays of res_x*res_y floats */
+sg_status sg_gen_camera_rays(const float eye[3], const float fwd[3], const float up[3], float fov_deg,
+                             uint32_t res_x, uint32_t res_y, float *const rays7[7]);
+/* rays with origins uniform in [lo,hi]^3 and directions uniform on the sphere */
+sg_status sg_gen_random_rays(uint64_t seed, uint32_t n, float lo, float hi, float *const rays7[7]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

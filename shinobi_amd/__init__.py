@@ -1,0 +1,448 @@
+"""shinobi_amd -- MI355X (gfx950) grid-acceleration hot path of Shinobi.
+
+Python mirror of the reference's host-side interface for this path, over the
+C ABI of libshinobi_grid.so (include/shinobi_grid.h):
+
+* ``FaceSoup`` -- primitive storage (src/Control/Structure/FaceSoup.h:41-247):
+  SoA triangles resident on the device; ``indices`` of a built grid map
+  references to triangle ids.
+* ``FastSimpleGridBuilder`` / ``FastTwoLevelGridBuilder`` -- the builder
+  concept ``init / build / rebuild``
+  (src/Control/Algorithm/FastSimpleGridBuilder.h:54-452,
+  src/Control/Algorithm/FastTwoLevelGridBuilder.h:56-740), producing a
+  ``SimpleGrid`` / ``TwoLevelGrid``.
+* ``Grid.trace`` -- the trace kernel contract
+  (src/Engine/RayTracingKernels.h:37-136): rays in, (triangle id, t) out.
+
+All arrays exchanged with the host are numpy; device memory is owned by
+``DeviceBuffer`` objects.  Everything runs on the GPU through the HIP
+library; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import ShinobiError, check
+
+MISS = 0xFFFFFFFF
+FLT_MAX = np.float32(3.402823466e38)
+
+__all__ = [
+    "Context", "DeviceBuffer", "FaceSoup", "FastSimpleGridBuilder", "FastTwoLevelGridBuilder", "Grid",
+    "GridInfo", "RayBatch", "ShinobiError", "MISS", "FLT_MAX", "gen_soup", "gen_heightfield",
+    "gen_cluster", "gen_camera_rays", "gen_random_rays", "heightfield_height", "build_params",
+    "device_count", "lib",
+]
+
+
+def lib() -> C.CDLL:
+    return L.load()
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    lib().sg_device_count(C.byref(n))
+    return n.value
+
+
+# --------------------------------------------------------------------- memory
+class Context:
+    """A device context (sg_ctx) plus the stream the host API enqueues on."""
+
+    def __init__(self, device: int = 0, own_stream: bool = True):
+        self._l = lib()
+        h = C.c_void_p()
+        check(self._l.sg_ctx_create(device, C.byref(h)), "sg_ctx_create")
+        self.h = h
+        self.device = device
+        self.stream = C.c_void_p()
+        if own_stream:
+            check(self._l.sg_stream_create(self.h, C.byref(self.stream)), "sg_stream_create")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            if self.stream:
+                self._l.sg_stream_sync(self.h, self.stream)
+                self._l.sg_stream_destroy(self.h, self.stream)
+                self.stream = C.c_void_p()
+            self._l.sg_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self) -> None:
+        check(self._l.sg_stream_sync(self.h, self.stream), "sg_stream_sync")
+
+    def alloc(self, nbytes: int) -> "DeviceBuffer":
+        return DeviceBuffer(self, nbytes)
+
+    def upload(self, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = DeviceBuffer(self, a.nbytes)
+        b.upload(a)
+        return b
+
+    def event(self) -> "Event":
+        return Event(self)
+
+
+class Event:
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        check(ctx._l.sg_event_create(ctx.h, C.byref(self.h)), "sg_event_create")
+
+    def record(self, stream=None) -> None:
+        check(self.ctx._l.sg_event_record(self.ctx.h, self.h, self.ctx.stream if stream is None else stream),
+              "sg_event_record")
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        check(self.ctx._l.sg_event_elapsed_ms(self.ctx.h, self.h, end.h, C.byref(ms)), "sg_event_elapsed_ms")
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx._l.sg_event_destroy(self.ctx.h, self.h)
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """A device allocation (sg_malloc); freed when the object is collected."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        check(ctx._l.sg_malloc(ctx.h, max(self.nbytes, 1), C.byref(self.ptr)), "sg_malloc")
+
+    @property
+    def addr(self) -> int:
+        return int(self.ptr.value)
+
+    def at(self, byte_offset: int) -> C.c_void_p:
+        return C.c_void_p(self.addr + int(byte_offset))
+
+    def upload(self, a: np.ndarray, byte_offset: int = 0) -> None:
+        a = np.ascontiguousarray(a)
+        assert byte_offset + a.nbytes <= self.nbytes
+        check(self.ctx._l.sg_memcpy_htod(self.ctx.h, self.at(byte_offset), a.ctypes.data_as(C.c_void_p),
+                                         a.nbytes, self.ctx.stream), "sg_memcpy_htod")
+
+    def download(self, dtype, count: int | None = None, byte_offset: int = 0) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - byte_offset) // dtype.itemsize
+        out = np.empty(count, dtype)
+        assert byte_offset + out.nbytes <= self.nbytes
+        check(self.ctx._l.sg_memcpy_dtoh(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.at(byte_offset),
+                                         out.nbytes, self.ctx.stream), "sg_memcpy_dtoh")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            self.ctx._l.sg_free(self.ctx.h, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# --------------------------------------------------------------------- rays
+class RayBatch:
+    """SoA rays (ox,oy,oz,dx,dy,dz,tmax) resident on the device."""
+
+    def __init__(self, ctx: Context, rays7: np.ndarray | None = None, capacity: int | None = None):
+        self.ctx = ctx
+        if rays7 is not None:
+            rays7 = np.ascontiguousarray(rays7, dtype=np.float32)
+            assert rays7.shape[0] == 7
+            self.n = rays7.shape[1]
+        else:
+            self.n = int(capacity)
+        self.cap = self.n
+        self.buf = DeviceBuffer(ctx, 7 * max(self.cap, 1) * 4)
+        if rays7 is not None and self.n:
+            self.buf.upload(rays7)
+
+    def field(self, k: int) -> C.c_void_p:
+        return self.buf.at(4 * k * self.cap)
+
+    def struct(self) -> L.Rays:
+        r = L.Rays()
+        for i in range(3):
+            r.o[i] = self.field(i)
+            r.d[i] = self.field(3 + i)
+        r.tmax = self.field(6)
+        return r
+
+    def download(self, count: int | None = None) -> np.ndarray:
+        a = self.buf.download(np.float32, 7 * self.cap).reshape(7, self.cap)
+        return a[:, : (self.n if count is None else count)].copy()
+
+
+# -------------------------------------------------------------------- storage
+class FaceSoup:
+    """Primitive storage: SoA triangles on the device (sg_scene)."""
+
+    def __init__(self, ctx: Context, soa9: np.ndarray):
+        self.ctx = ctx
+        soa9 = np.ascontiguousarray(soa9, dtype=np.float32)
+        assert soa9.ndim == 2 and soa9.shape[0] == 9
+        self.n = int(soa9.shape[1])
+        ptrs = L.Float9(*[soa9[i].ctypes.data for i in range(9)])
+        h = C.c_void_p()
+        check(ctx._l.sg_scene_create(ctx.h, ptrs, self.n, 0, ctx.stream, C.byref(h)), "sg_scene_create")
+        self.h = h
+
+    def upload_vertices(self, soa9: np.ndarray) -> None:
+        """FaceSoup::uploadVertices: new positions, same topology."""
+        soa9 = np.ascontiguousarray(soa9, dtype=np.float32)
+        assert soa9.shape == (9, self.n)
+        ptrs = L.Float9(*[soa9[i].ctypes.data for i in range(9)])
+        check(self.ctx._l.sg_scene_update(self.h, ptrs, 0, self.ctx.stream), "sg_scene_update")
+
+    def device_soa(self) -> list[int]:
+        out = L.Float9()
+        check(self.ctx._l.sg_scene_device_soa(self.h, out), "sg_scene_device_soa")
+        return [int(x) for x in out]
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.ctx._l.sg_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------- grids
+def build_params(kind: int, **kw) -> L.BuildParams:
+    p = L.BuildParams()
+    lib().sg_build_params_default(C.byref(p), kind)
+    for k, v in kw.items():
+        if k == "lam":
+            k = "lambda_"
+        if k == "res_override":
+            for i in range(3):
+                p.res_override[i] = int(v[i])
+        else:
+            setattr(p, k, v)
+    return p
+
+
+@dataclass
+class GridInfo:
+    kind: int
+    status: int
+    res: tuple
+    bmin: np.ndarray
+    bmax: np.ndarray
+    cell_size: np.ndarray
+    cell_size_rcp: np.ndarray
+    build_rcp: np.ndarray
+    n_tris: int
+    C1: int
+    P1: int
+    P1_valid: int
+    C2: int
+    P2: int
+    P2_valid: int
+    caps: tuple
+    device_bytes: int
+
+
+ARRAYS = {
+    "top_keys": L.SG_ARRAY_TOP_KEYS, "top_vals": L.SG_ARRAY_TOP_VALS, "top_ranges": L.SG_ARRAY_TOP_RANGES,
+    "top_cells": L.SG_ARRAY_TOP_CELLS, "leaf_keys": L.SG_ARRAY_LEAF_KEYS, "leaf_vals": L.SG_ARRAY_LEAF_VALS,
+    "leaves": L.SG_ARRAY_LEAVES, "indices": L.SG_ARRAY_INDICES, "tri_counts": L.SG_ARRAY_TRI_COUNTS,
+    "leaf_counts": L.SG_ARRAY_LEAF_COUNTS, "pair_counts": L.SG_ARRAY_PAIR_COUNTS,
+}
+
+
+class Grid:
+    """A built SimpleGrid or TwoLevelGrid (sg_grid) over a FaceSoup."""
+
+    def __init__(self, ctx: Context, scene: FaceSoup, h: C.c_void_p, params: L.BuildParams):
+        self.ctx, self.scene, self.h, self.params = ctx, scene, h, params
+
+    def info(self) -> GridInfo:
+        gi = L.GridInfo()
+        check(self.ctx._l.sg_grid_get_info(self.h, C.byref(gi)), "sg_grid_get_info")
+        f3 = lambda a: np.array(list(a), np.float32)  # noqa: E731
+        return GridInfo(gi.kind, gi.status, tuple(gi.res), f3(gi.bmin), f3(gi.bmax), f3(gi.cell_size),
+                        f3(gi.cell_size_rcp), f3(gi.build_rcp), gi.n_tris, gi.C1, gi.P1, gi.P1_valid, gi.C2,
+                        gi.P2, gi.P2_valid, (gi.cap_C1, gi.cap_P1, gi.cap_C2, gi.cap_P2), gi.device_bytes)
+
+    def download(self, which: str) -> np.ndarray:
+        inf = self.info()
+        sizes = {"top_keys": inf.P1, "top_vals": inf.P1, "top_ranges": 2 * inf.C1, "top_cells": 2 * inf.C1,
+                 "leaf_keys": inf.P2, "leaf_vals": inf.P2, "leaves": 2 * inf.C2,
+                 "indices": inf.P2 if inf.kind == L.SG_GRID_TWO_LEVEL else inf.P1,
+                 "tri_counts": inf.n_tris, "leaf_counts": inf.C1, "pair_counts": inf.P1}
+        out = np.zeros(max(sizes[which], 1), np.uint32)
+        check(self.ctx._l.sg_grid_download(self.h, ARRAYS[which], out.ctypes.data_as(C.c_void_p), out.nbytes),
+              f"sg_grid_download({which})")
+        return out[: sizes[which]]
+
+    def rebuild(self, free_resolution: bool = False) -> None:
+        f = self.ctx._l.sg_rebuild_free_res if free_resolution else self.ctx._l.sg_rebuild
+        check(f(self.ctx.h, self.scene.h, self.h, self.ctx.stream), "sg_rebuild")
+
+    def trace(self, rays: RayBatch, prim: DeviceBuffer | None = None, t: DeviceBuffer | None = None,
+              stats: DeviceBuffer | None = None, d_count: C.c_void_p | None = None, n: int | None = None):
+        n = rays.n if n is None else n
+        prim = prim or DeviceBuffer(self.ctx, 4 * max(n, 1))
+        t = t or DeviceBuffer(self.ctx, 4 * max(n, 1))
+        r = rays.struct()
+        check(self.ctx._l.sg_trace_closest(self.ctx.h, self.h, C.byref(r), n, d_count, prim.ptr, t.ptr,
+                                           stats.ptr if stats is not None else None, self.ctx.stream),
+              "sg_trace_closest")
+        return prim, t
+
+    def trace_host(self, rays7: np.ndarray, with_stats: bool = False):
+        """Convenience: upload rays, trace, download (prim, t[, stats])."""
+        rb = RayBatch(self.ctx, rays7)
+        stats = DeviceBuffer(self.ctx, 16 * max(rb.n, 1)) if with_stats else None
+        prim, t = self.trace(rb, stats=stats)
+        out = (prim.download(np.uint32, rb.n), t.download(np.float32, rb.n))
+        if with_stats:
+            out = out + (stats.download(np.uint32, 4 * rb.n).reshape(rb.n, 4),)
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.ctx._l.sg_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Builder:
+    KIND = L.SG_GRID_UNIFORM
+
+    def __init__(self, ctx: Context, **params):
+        self.ctx = ctx
+        self.params = build_params(self.KIND, **params)
+        self.scene: FaceSoup | None = None
+        self.grid: Grid | None = None
+
+    def init(self, scene: FaceSoup) -> None:
+        """Builder::init -- bind the storage (bounds and resolution are
+        computed on the device by build)."""
+        self.scene = scene
+
+    def build(self) -> Grid:
+        assert self.scene is not None, "init() first"
+        h = C.c_void_p()
+        check(self.ctx._l.sg_build(self.ctx.h, self.scene.h, C.byref(self.params), self.ctx.stream, C.byref(h)),
+              "sg_build")
+        self.grid = Grid(self.ctx, self.scene, h, self.params)
+        return self.grid
+
+    def rebuild(self) -> Grid:
+        assert self.grid is not None, "build() first"
+        self.grid.rebuild()
+        return self.grid
+
+
+class FastSimpleGridBuilder(_Builder):
+    """Uniform grid builder (src/Control/Algorithm/FastSimpleGridBuilder.h:43-486)."""
+    KIND = L.SG_GRID_UNIFORM
+
+
+class FastTwoLevelGridBuilder(_Builder):
+    """Two-level grid builder (src/Control/Algorithm/FastTwoLevelGridBuilder.h:45-783)."""
+    KIND = L.SG_GRID_TWO_LEVEL
+
+
+def bounce(ctx: Context, scene: FaceSoup, rays: RayBatch, prim: DeviceBuffer, t: DeviceBuffer, seed: int,
+           out: RayBatch, src: DeviceBuffer | None = None, count: DeviceBuffer | None = None) -> DeviceBuffer:
+    """One cosine-weighted diffuse bounce per hit, compacted; returns the
+    device counter holding the number of bounce rays."""
+    count = count or DeviceBuffer(ctx, 4)
+    r = rays.struct()
+    o = out.struct()
+    check(ctx._l.sg_bounce(ctx.h, scene.h, C.byref(r), prim.ptr, t.ptr, rays.n, seed, C.byref(o),
+                           src.ptr if src is not None else None, count.ptr, ctx.stream), "sg_bounce")
+    return count
+
+
+def scan_u32(ctx: Context, a: np.ndarray, inclusive: bool = False) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.uint32)
+    din = ctx.upload(a)
+    dout = DeviceBuffer(ctx, a.nbytes)
+    check(ctx._l.sg_scan_u32(ctx.h, din.ptr, dout.ptr, a.size, int(inclusive), ctx.stream), "sg_scan_u32")
+    return dout.download(np.uint32, a.size)
+
+
+def sort_pairs(ctx: Context, keys: np.ndarray, vals: np.ndarray, key_bits: int = 32, radix_bits: int = 8):
+    keys = np.ascontiguousarray(keys, np.uint32)
+    vals = np.ascontiguousarray(vals, np.uint32)
+    dk, dv = ctx.upload(keys), ctx.upload(vals)
+    check(ctx._l.sg_sort_pairs(ctx.h, dk.ptr, dv.ptr, keys.size, key_bits, radix_bits, ctx.stream),
+          "sg_sort_pairs")
+    return dk.download(np.uint32, keys.size), dv.download(np.uint32, keys.size)
+
+
+# ----------------------------------------------------------------- generators
+def _soa(n: int, rows: int):
+    a = np.empty((rows, n), np.float32)
+    ptr_t = L.Float9 if rows == 9 else L.Float7
+    return a, ptr_t(*[a[i].ctypes.data for i in range(rows)])
+
+
+def gen_soup(seed: int, n: int, edge: float = 0.01) -> np.ndarray:
+    a, p = _soa(n, 9)
+    check(lib().sg_gen_soup(seed, n, edge, p), "sg_gen_soup")
+    return a
+
+
+def gen_heightfield(seed: int, grid: int = 1024, amplitude: float = 0.2) -> np.ndarray:
+    n = 2 * (grid - 1) ** 2
+    a, p = _soa(n, 9)
+    check(lib().sg_gen_heightfield(seed, grid, amplitude, p), "sg_gen_heightfield")
+    return a
+
+
+def heightfield_height(seed: int, x: float, y: float, amplitude: float = 0.2) -> float:
+    return float(lib().sg_heightfield_height(seed, x, y, amplitude))
+
+
+def gen_cluster(seed: int, n_small: int = 500_000, n_large: int = 2_000) -> np.ndarray:
+    a, p = _soa(n_small + n_large, 9)
+    check(lib().sg_gen_cluster(seed, n_small, n_large, p), "sg_gen_cluster")
+    return a
+
+
+def gen_camera_rays(eye, fwd, up, fov_deg: float, res_x: int, res_y: int) -> np.ndarray:
+    a, p = _soa(res_x * res_y, 7)
+    f3 = C.c_float * 3
+    check(lib().sg_gen_camera_rays(f3(*eye), f3(*fwd), f3(*up), fov_deg, res_x, res_y, p), "sg_gen_camera_rays")
+    return a
+
+
+def gen_random_rays(seed: int, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+    a, p = _soa(n, 7)
+    check(lib().sg_gen_random_rays(seed, n, lo, hi, p), "sg_gen_random_rays")
+    return a

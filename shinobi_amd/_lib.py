@@ -1,0 +1,145 @@
+"""ctypes declarations of the C ABI in include/shinobi_grid.h.
+
+The library is loaded from shinobi_amd/lib/libshinobi_grid.so (built in-tree
+by shinobi_amd/build.py).  There is no fallback: if the library is missing
+the import fails with an error naming the build command.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libshinobi_grid.so")
+
+SG_OK = 0
+STATUS_NAMES = {
+    0: "SG_OK", 1: "SG_ERR_INVALID_ARG", 2: "SG_ERR_INVALID_SCENE", 3: "SG_ERR_OUT_OF_MEMORY",
+    4: "SG_ERR_HIP", 5: "SG_ERR_CAPACITY", 6: "SG_ERR_LEAF_RES", 7: "SG_ERR_TOO_MANY_CELLS",
+    8: "SG_ERR_NEG_EXTENT", 9: "SG_ERR_TIMEOUT", 10: "SG_ERR_BUDGET", 11: "SG_ERR_NO_DEVICE",
+}
+
+SG_GRID_UNIFORM = 0
+SG_GRID_TWO_LEVEL = 1
+
+(SG_ARRAY_TOP_KEYS, SG_ARRAY_TOP_VALS, SG_ARRAY_TOP_RANGES, SG_ARRAY_TOP_CELLS, SG_ARRAY_LEAF_KEYS,
+ SG_ARRAY_LEAF_VALS, SG_ARRAY_LEAVES, SG_ARRAY_INDICES, SG_ARRAY_TRI_COUNTS, SG_ARRAY_LEAF_COUNTS,
+ SG_ARRAY_PAIR_COUNTS) = range(11)
+
+c_float_p = C.POINTER(C.c_float)
+c_u32_p = C.POINTER(C.c_uint32)
+Float9 = C.c_void_p * 9
+Float7 = C.c_void_p * 7
+
+
+class BuildParams(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("lambda_", C.c_float),
+        ("top_divisor", C.c_int),
+        ("lambda_leaf", C.c_float),
+        ("leaf_min_refs", C.c_uint32),
+        ("res_override", C.c_int * 3),
+        ("radix_bits", C.c_int),
+        ("capacity_slack", C.c_float),
+        ("keep_debug", C.c_int),
+    ]
+
+
+class GridInfo(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("status", C.c_int),
+        ("res", C.c_int * 3),
+        ("bmin", C.c_float * 3),
+        ("bmax", C.c_float * 3),
+        ("cell_size", C.c_float * 3),
+        ("cell_size_rcp", C.c_float * 3),
+        ("build_rcp", C.c_float * 3),
+        ("n_tris", C.c_uint32),
+        ("C1", C.c_uint32), ("P1", C.c_uint32), ("P1_valid", C.c_uint32),
+        ("C2", C.c_uint32), ("P2", C.c_uint32), ("P2_valid", C.c_uint32),
+        ("cap_C1", C.c_uint32), ("cap_P1", C.c_uint32), ("cap_C2", C.c_uint32), ("cap_P2", C.c_uint32),
+        ("device_bytes", C.c_uint64),
+    ]
+
+
+class Rays(C.Structure):
+    _fields_ = [("o", C.c_void_p * 3), ("d", C.c_void_p * 3), ("tmax", C.c_void_p)]
+
+
+# name -> (restype, argtypes); the authoritative list of exported entry points
+SIGNATURES = {
+    "sg_status_str": (C.c_char_p, [C.c_int]),
+    "sg_build_params_default": (None, [C.POINTER(BuildParams), C.c_int]),
+    "sg_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "sg_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "sg_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "sg_scene_create": (C.c_int, [C.c_void_p, Float9, C.c_uint32, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "sg_scene_update": (C.c_int, [C.c_void_p, Float9, C.c_int, C.c_void_p]),
+    "sg_scene_destroy": (C.c_int, [C.c_void_p]),
+    "sg_scene_size": (C.c_uint32, [C.c_void_p]),
+    "sg_scene_device_soa": (C.c_int, [C.c_void_p, Float9]),
+    "sg_build": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BuildParams), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "sg_rebuild": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sg_rebuild_free_res": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sg_grid_get_info": (C.c_int, [C.c_void_p, C.POINTER(GridInfo)]),
+    "sg_grid_download": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
+    "sg_grid_destroy": (C.c_int, [C.c_void_p]),
+    "sg_trace_closest": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_uint32, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sg_bounce": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_uint32,
+                            C.c_uint64, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sg_scan_u32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]),
+    "sg_sort_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p]),
+    "sg_malloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "sg_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sg_memcpy_htod": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "sg_memcpy_dtoh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "sg_stream_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "sg_stream_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sg_stream_sync": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sg_event_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "sg_event_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sg_event_record": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sg_event_elapsed_ms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
+    "sg_gen_soup": (C.c_int, [C.c_uint64, C.c_uint32, C.c_float, Float9]),
+    "sg_gen_heightfield": (C.c_int, [C.c_uint64, C.c_uint32, C.c_float, Float9]),
+    "sg_heightfield_height": (C.c_float, [C.c_uint64, C.c_float, C.c_float, C.c_float]),
+    "sg_gen_cluster": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, Float9]),
+    "sg_gen_camera_rays": (C.c_int, [C.c_float * 3, C.c_float * 3, C.c_float * 3, C.c_float, C.c_uint32,
+                                     C.c_uint32, Float7]),
+    "sg_gen_random_rays": (C.c_int, [C.c_uint64, C.c_uint32, C.c_float, C.c_float, Float7]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `python shinobi_amd/build.py`")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class ShinobiError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        name = STATUS_NAMES.get(status, str(status))
+        msg = load().sg_status_str(status).decode()
+        super().__init__(f"{what}: {name} ({msg})")
+
+
+def check(status: int, what: str) -> None:
+    if status != SG_OK:
+        raise ShinobiError(status, what)

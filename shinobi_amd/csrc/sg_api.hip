@@ -1,0 +1,748 @@
+// sg_api.hip -- the C ABI (include/shinobi_grid.h): contexts, scenes, grid
+// builds, traversal and device plumbing on top of the gfx950 kernels.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "../../include/shinobi_grid.h"
+#include "sg_kernels.hpp"
+
+using namespace sg;
+
+struct sg_ctx {
+    int device;
+    // scratch for the standalone primitives (sg_scan_u32, sg_sort_pairs, sg_bounce)
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+struct sg_scene {
+    sg_ctx* ctx;
+    uint32_t n;
+    float* soa = nullptr;     // 9*n floats
+    float4* rec = nullptr;    // 3*n float4 traversal records
+};
+
+struct sg_grid {
+    sg_ctx* ctx;
+    sg_scene* scene;
+    sg_build_params p;
+    int kind;
+    uint32_t n_tris;
+    int res[3];
+    // device memory: one allocation carved into the arrays below
+    char* pool = nullptr;
+    size_t pool_bytes = 0;
+    GridHeader* hdr = nullptr;
+    uint32_t cap_C1 = 0, cap_P1 = 0, cap_C2 = 0, cap_P2 = 0;
+    uint32_t *k1[2] = {nullptr, nullptr}, *v1[2] = {nullptr, nullptr};
+    uint32_t *ranges1 = nullptr, *cells1 = nullptr;
+    uint32_t *k2[2] = {nullptr, nullptr}, *v2[2] = {nullptr, nullptr};
+    uint32_t* leaves = nullptr;
+    uint32_t *dbg_tri = nullptr, *dbg_leafc = nullptr, *dbg_pair = nullptr;
+    uint64_t *lb_top = nullptr, *lb_cells = nullptr, *lb_leaf = nullptr;
+    size_t lb_words = 0;
+    uint32_t *sort1 = nullptr, *sort2 = nullptr;
+    size_t sort1_words = 0, sort2_words = 0;
+    int key_bits1 = 1, key_bits2 = 1, passes1 = 1, passes2 = 1, rb = 8;
+    hipStream_t stream = nullptr;
+};
+
+#define HIPCHK(x)                                   \
+    do {                                            \
+        hipError_t e_ = (x);                        \
+        if (e_ != hipSuccess) return hip_status(e_); \
+    } while (0)
+
+static sg_status hip_status(hipError_t e) {
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return SG_ERR_OUT_OF_MEMORY;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return SG_ERR_NO_DEVICE;
+    return SG_ERR_HIP;
+}
+
+static sg_status status_from_bits(uint32_t b) {
+    if (b & ST_INVALID) return SG_ERR_INVALID_SCENE;
+    if (b & ST_TOO_MANY_CELLS) return SG_ERR_TOO_MANY_CELLS;
+    if (b & (ST_CAP_P1 | ST_CAP_C1 | ST_CAP_C2 | ST_CAP_P2)) return SG_ERR_CAPACITY;
+    if (b & ST_SPIN) return SG_ERR_TIMEOUT;
+    if (b & ST_NEG_EXTENT) return SG_ERR_NEG_EXTENT;
+    if (b & ST_LEAF_RES) return SG_ERR_LEAF_RES;
+    return SG_OK;
+}
+
+static int bit_length(uint64_t x) {
+    int b = 0;
+    while (x) {
+        ++b;
+        x >>= 1;
+    }
+    return b < 1 ? 1 : b;
+}
+
+extern "C" {
+
+const char* sg_status_str(sg_status s) {
+    switch (s) {
+        case SG_OK: return "ok";
+        case SG_ERR_INVALID_ARG: return "invalid argument";
+        case SG_ERR_INVALID_SCENE: return "invalid scene (empty, zero-extent bounds or resolution < 1)";
+        case SG_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case SG_ERR_HIP: return "HIP runtime error";
+        case SG_ERR_CAPACITY: return "rebuild exceeded the grid's buffer capacity";
+        case SG_ERR_LEAF_RES: return "leaf resolution outside [1,255]";
+        case SG_ERR_TOO_MANY_CELLS: return "too many cells (>= 2^24)";
+        case SG_ERR_NEG_EXTENT: return "negative leaf extent";
+        case SG_ERR_TIMEOUT: return "device-side wait exceeded its bound";
+        case SG_ERR_BUDGET: return "request exceeds the HBM budget";
+        case SG_ERR_NO_DEVICE: return "no HIP device";
+        default: return "unknown status";
+    }
+}
+
+void sg_build_params_default(sg_build_params* p, int kind) {
+    memset(p, 0, sizeof(*p));
+    p->kind = kind;
+    p->lambda = 5.0f;
+    p->top_divisor = kind == SG_GRID_TWO_LEVEL ? 6 : 1;
+    p->lambda_leaf = 1.2f;
+    p->leaf_min_refs = 16;
+    p->radix_bits = 8;
+    p->capacity_slack = 1.25f;
+}
+
+sg_status sg_device_count(int* out) {
+    if (!out) return SG_ERR_INVALID_ARG;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *out = e == hipSuccess ? n : 0;
+    return SG_OK;
+}
+
+sg_status sg_ctx_create(int device, sg_ctx** out) {
+    if (!out) return SG_ERR_INVALID_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return SG_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(device));
+    sg_ctx* c = new sg_ctx();
+    c->device = device;
+    *out = c;
+    return SG_OK;
+}
+
+sg_status sg_ctx_destroy(sg_ctx* ctx) {
+    if (!ctx) return SG_ERR_INVALID_ARG;
+    hipSetDevice(ctx->device);
+    if (ctx->scratch) hipFree(ctx->scratch);
+    delete ctx;
+    return SG_OK;
+}
+
+static sg_status ctx_scratch(sg_ctx* ctx, size_t bytes, void** out) {
+    if (ctx->scratch_bytes < bytes) {
+        if (ctx->scratch) {
+            HIPCHK(hipDeviceSynchronize());
+            hipFree(ctx->scratch);
+            ctx->scratch = nullptr;
+            ctx->scratch_bytes = 0;
+        }
+        size_t b = std::max(bytes, (size_t)1 << 20);
+        HIPCHK(hipMalloc(&ctx->scratch, b));
+        ctx->scratch_bytes = b;
+    }
+    *out = ctx->scratch;
+    return SG_OK;
+}
+
+// ------------------------------------------------------------------ scenes
+static TriSoA scene_soa(const sg_scene* s) {
+    TriSoA t;
+    for (int i = 0; i < 9; ++i) t.v[i] = s->soa + (size_t)i * s->n;
+    return t;
+}
+
+static sg_status scene_upload(sg_scene* s, const float* const soa9[9], int on_dev, hipStream_t st) {
+    const size_t b = (size_t)s->n * sizeof(float);
+    for (int i = 0; i < 9; ++i) {
+        if (!soa9[i]) return SG_ERR_INVALID_ARG;
+        HIPCHK(hipMemcpyAsync(s->soa + (size_t)i * s->n, soa9[i], b,
+                              on_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    }
+    launch_prep_tris(scene_soa(s), s->n, s->rec, st);
+    HIPCHK(hipGetLastError());
+    if (!on_dev) HIPCHK(hipStreamSynchronize(st));
+    return SG_OK;
+}
+
+sg_status sg_scene_create(sg_ctx* ctx, const float* const soa9[9], uint32_t n, int src_on_device, void* stream,
+                          sg_scene** out) {
+    if (!ctx || !soa9 || !out || n == 0) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    sg_scene* s = new sg_scene();
+    s->ctx = ctx;
+    s->n = n;
+    if (hipMalloc(&s->soa, (size_t)9 * n * sizeof(float)) != hipSuccess ||
+        hipMalloc(&s->rec, (size_t)3 * n * sizeof(float4)) != hipSuccess) {
+        if (s->soa) hipFree(s->soa);
+        delete s;
+        return SG_ERR_OUT_OF_MEMORY;
+    }
+    sg_status r = scene_upload(s, soa9, src_on_device, (hipStream_t)stream);
+    if (r != SG_OK) {
+        hipFree(s->soa);
+        hipFree(s->rec);
+        delete s;
+        return r;
+    }
+    *out = s;
+    return SG_OK;
+}
+
+sg_status sg_scene_update(sg_scene* s, const float* const soa9[9], int src_on_device, void* stream) {
+    if (!s || !soa9) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(s->ctx->device));
+    return scene_upload(s, soa9, src_on_device, (hipStream_t)stream);
+}
+
+sg_status sg_scene_destroy(sg_scene* s) {
+    if (!s) return SG_ERR_INVALID_ARG;
+    hipSetDevice(s->ctx->device);
+    hipFree(s->soa);
+    hipFree(s->rec);
+    delete s;
+    return SG_OK;
+}
+
+uint32_t sg_scene_size(const sg_scene* s) { return s ? s->n : 0u; }
+
+sg_status sg_scene_device_soa(const sg_scene* s, const float* out9[9]) {
+    if (!s || !out9) return SG_ERR_INVALID_ARG;
+    for (int i = 0; i < 9; ++i) out9[i] = s->soa + (size_t)i * s->n;
+    return SG_OK;
+}
+
+// ------------------------------------------------------------------- grids
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// lay out (or measure, pool == nullptr) every array of the grid
+static size_t grid_layout(sg_grid* g, char* pool) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char* {
+        off = align_up(off, 256);
+        char* p = pool ? pool + off : nullptr;
+        off += align_up(bytes, 16);
+        return p;
+    };
+    const bool two = g->kind == SG_GRID_TWO_LEVEL;
+    const uint32_t N = g->n_tris;
+    g->hdr = (GridHeader*)take(sizeof(GridHeader));
+    for (int b = 0; b < 2; ++b) {
+        g->k1[b] = (uint32_t*)take((size_t)g->cap_P1 * 4 + 16);
+        g->v1[b] = (uint32_t*)take((size_t)g->cap_P1 * 4 + 16);
+    }
+    g->ranges1 = (uint32_t*)take((size_t)g->cap_C1 * 8 + 16);
+    g->cells1 = two ? (uint32_t*)take((size_t)g->cap_C1 * 8 + 16) : nullptr;
+    if (two) {
+        for (int b = 0; b < 2; ++b) {
+            g->k2[b] = (uint32_t*)take((size_t)g->cap_P2 * 4 + 16);
+            g->v2[b] = (uint32_t*)take((size_t)g->cap_P2 * 4 + 16);
+        }
+        g->leaves = (uint32_t*)take((size_t)g->cap_C2 * 8 + 16);
+    }
+    if (g->p.keep_debug) {
+        g->dbg_tri = (uint32_t*)take((size_t)N * 4 + 16);
+        g->dbg_leafc = two ? (uint32_t*)take((size_t)g->cap_C1 * 4 + 16) : nullptr;
+        g->dbg_pair = two ? (uint32_t*)take((size_t)g->cap_P1 * 4 + 16) : nullptr;
+    } else {
+        g->dbg_tri = g->dbg_leafc = g->dbg_pair = nullptr;
+    }
+    const size_t t_top = ((size_t)N + kBlock - 1) / kBlock;
+    const size_t t_cells = ((size_t)g->cap_C1 + kBlock - 1) / kBlock;
+    const size_t t_leaf = ((size_t)g->cap_P1 + kBlock - 1) / kBlock;
+    g->lb_words = align_up(t_top, 2) + (two ? align_up(t_cells, 2) + align_up(t_leaf, 2) : 0);
+    char* lb = take(g->lb_words * 8);
+    g->lb_top = (uint64_t*)lb;
+    g->lb_cells = lb ? g->lb_top + align_up(t_top, 2) : nullptr;
+    g->lb_leaf = lb ? g->lb_cells + align_up(t_cells, 2) : nullptr;
+    g->key_bits1 = bit_length(g->cap_C1);
+    g->passes1 = sort_passes(g->key_bits1, g->rb);
+    g->sort1_words = sort_scratch_words(g->cap_P1, g->key_bits1, g->rb);
+    g->sort1 = (uint32_t*)take(g->sort1_words * 4);
+    if (two) {
+        g->key_bits2 = bit_length(g->cap_C2);
+        g->passes2 = sort_passes(g->key_bits2, g->rb);
+        g->sort2_words = sort_scratch_words(g->cap_P2, g->key_bits2, g->rb);
+        g->sort2 = (uint32_t*)take(g->sort2_words * 4);
+    }
+    return align_up(off, 256);
+}
+
+static sg_status grid_alloc(sg_grid* g) {
+    const size_t bytes = grid_layout(g, nullptr);
+    if (g->pool && g->pool_bytes >= bytes) {
+        grid_layout(g, g->pool);
+        return SG_OK;
+    }
+    if (g->pool) {
+        HIPCHK(hipStreamSynchronize(g->stream));
+        hipFree(g->pool);
+        g->pool = nullptr;
+        g->pool_bytes = 0;
+    }
+    char* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return SG_ERR_OUT_OF_MEMORY;
+    g->pool = p;
+    g->pool_bytes = bytes;
+    grid_layout(g, g->pool);
+    return SG_OK;
+}
+
+static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st) {
+    const bool two = g->kind == SG_GRID_TWO_LEVEL;
+    const TriSoA t = scene_soa(g->scene);
+    const uint32_t N = g->n_tris;
+    BuildArgs a;
+    a.lambda = g->p.lambda;
+    a.top_divisor = two ? g->p.top_divisor : 1;
+    a.lambda_leaf = g->p.lambda_leaf;
+    a.leaf_min_refs = g->p.leaf_min_refs;
+    for (int i = 0; i < 3; ++i) a.fixed_res[i] = fixed_res ? g->res[i] : 0;
+    a.cap_C1 = g->cap_C1;
+    a.cap_P1 = g->cap_P1;
+    a.cap_C2 = g->cap_C2;
+    a.cap_P2 = g->cap_P2;
+    a.keep_debug = g->p.keep_debug ? 1u : 0u;
+
+    ZeroSegs z;
+    z.count = 0;
+    auto seg = [&](void* p, size_t bytes) {
+        if (p && bytes) {
+            z.ptr[z.count] = p;
+            z.bytes[z.count] = align_up(bytes, 16);
+            ++z.count;
+        }
+    };
+    seg(g->lb_top, g->lb_words * 8);
+    seg(g->sort1, g->sort1_words * 4);
+    seg(g->ranges1, (size_t)g->cap_C1 * 8);
+    if (two) {
+        seg(g->sort2, g->sort2_words * 4);
+        seg(g->leaves, (size_t)g->cap_C2 * 8);
+    }
+    // per-pair leaf counts are only written below P1_valid
+    seg(g->dbg_pair, (size_t)g->cap_P1 * 4);
+    launch_init(g->hdr, N, z, st);
+    launch_bounds(t, N, g->hdr, st);
+    launch_top_emit(t, N, g->hdr, a, g->k1[0], g->v1[0], g->dbg_tri, g->lb_top, st);
+    launch_sort_pairs(g->k1[0], g->v1[0], g->k1[1], g->v1[1], &g->hdr->P1, g->cap_P1, g->key_bits1, g->rb, g->sort1,
+                      &g->hdr->tile_ctr[3], &g->hdr->status, st);
+    const int f1 = g->passes1 & 1;
+    launch_ranges(g->k1[f1], &g->hdr->P1, g->cap_P1, &g->hdr->C1, g->ranges1, &g->hdr->P1_valid, st);
+    if (!two) return;
+    launch_leaf_cells(g->hdr, a, g->ranges1, g->cells1, g->dbg_leafc, g->lb_cells, st);
+    launch_leaf_emit(t, g->hdr, a, g->k1[f1], g->v1[f1], g->cells1, g->k2[0], g->v2[0], g->dbg_pair, g->lb_leaf, st);
+    launch_sort_pairs(g->k2[0], g->v2[0], g->k2[1], g->v2[1], &g->hdr->P2, g->cap_P2, g->key_bits2, g->rb, g->sort2,
+                      &g->hdr->tile_ctr[3 + g->passes1], &g->hdr->status, st);
+    const int f2 = g->passes2 & 1;
+    launch_ranges(g->k2[f2], &g->hdr->P2, g->cap_P2, &g->hdr->C2, g->leaves, &g->hdr->P2_valid, st);
+}
+
+static sg_status read_header(sg_grid* g, GridHeader* h) {
+    HIPCHK(hipStreamSynchronize(g->stream));
+    HIPCHK(hipMemcpy(h, g->hdr, sizeof(GridHeader), hipMemcpyDeviceToHost));
+    return SG_OK;
+}
+
+sg_status sg_build(sg_ctx* ctx, sg_scene* scene, const sg_build_params* p, void* stream, sg_grid** out) {
+    if (!ctx || !scene || !p || !out) return SG_ERR_INVALID_ARG;
+    if (p->kind != SG_GRID_UNIFORM && p->kind != SG_GRID_TWO_LEVEL) return SG_ERR_INVALID_ARG;
+    if (p->radix_bits != 0 && p->radix_bits != 4 && p->radix_bits != 6 && p->radix_bits != 8)
+        return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    sg_grid* g = new sg_grid();
+    g->ctx = ctx;
+    g->scene = scene;
+    g->p = *p;
+    g->kind = p->kind;
+    g->n_tris = scene->n;
+    g->rb = p->radix_bits ? p->radix_bits : 8;
+    g->stream = st;
+    const float slack = p->capacity_slack >= 1.0f ? p->capacity_slack : 1.25f;
+
+    // 1. bounds and resolution (the one value the host needs to size the top level)
+    {
+        g->cap_C1 = 1;
+        g->cap_P1 = 1;
+        g->cap_C2 = 1;
+        g->cap_P2 = 1;
+        sg_status r = grid_alloc(g);
+        if (r != SG_OK) {
+            delete g;
+            return r;
+        }
+        ZeroSegs z;
+        z.count = 0;
+        launch_init(g->hdr, g->n_tris, z, st);
+        launch_bounds(scene_soa(scene), g->n_tris, g->hdr, st);
+        GridHeader h;
+        r = read_header(g, &h);
+        if (r != SG_OK) {
+            sg_grid_destroy(g);
+            return r;
+        }
+        float mn[3], mx[3], cs[3], rc[3], cr[3];
+        for (int i = 0; i < 3; ++i) {
+            mn[i] = fkey_inv(h.bmin_key[i]);
+            mx[i] = fkey_inv(h.bmax_key[i]);
+        }
+        const int two = p->kind == SG_GRID_TWO_LEVEL;
+        if (!derive_params(mn, mx, g->n_tris, p->lambda, two ? p->top_divisor : 1, p->res_override, g->res, cs, rc,
+                           cr)) {
+            sg_grid_destroy(g);
+            return SG_ERR_INVALID_SCENE;
+        }
+        const uint64_t C1 = (uint64_t)g->res[0] * g->res[1] * g->res[2];
+        if (C1 >= (1ull << 24)) {
+            sg_grid_destroy(g);
+            return SG_ERR_TOO_MANY_CELLS;
+        }
+        g->cap_C1 = (uint32_t)C1;
+        const uint64_t N = g->n_tris;
+        g->cap_P1 = (uint32_t)std::min<uint64_t>(N * 2 + 4096, 0x3FFFFFFFull);
+        g->cap_C2 = two ? (uint32_t)std::min<uint64_t>(C1 + N * 2 + 4096, 0x3FFFFFFFull) : 1;
+        g->cap_P2 = two ? (uint32_t)std::min<uint64_t>(N * 4 + 4096, 0x3FFFFFFFull) : 1;
+    }
+    // 2. full build; grow the buffers to the measured sizes until it fits
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        sg_status r = grid_alloc(g);
+        if (r != SG_OK) {
+            sg_grid_destroy(g);
+            return r;
+        }
+        enqueue_build(g, true, st);
+        HIPCHK(hipGetLastError());
+        GridHeader h;
+        r = read_header(g, &h);
+        if (r != SG_OK) {
+            sg_grid_destroy(g);
+            return r;
+        }
+        auto grow = [&](uint32_t need) {
+            return (uint32_t)std::min<uint64_t>((uint64_t)(need * (double)slack) + 4096, 0x3FFFFFFFull);
+        };
+        const uint32_t cap_bits = ST_CAP_P1 | ST_CAP_C1 | ST_CAP_C2 | ST_CAP_P2;
+        if (!(h.status & cap_bits)) {
+            // fits: keep it if every buffer has the requested headroom for
+            // rebuilds of a moving scene, else grow once and build again
+            const bool two = g->kind == SG_GRID_TWO_LEVEL;
+            bool again = false;
+            if ((double)g->cap_P1 < h.need_P1 * (double)slack) { g->cap_P1 = grow(h.need_P1); again = true; }
+            if (two && (double)g->cap_C2 < h.need_C2 * (double)slack) { g->cap_C2 = grow(h.need_C2); again = true; }
+            if (two && (double)g->cap_P2 < h.need_P2 * (double)slack) { g->cap_P2 = grow(h.need_P2); again = true; }
+            if (!again) {
+                *out = g;
+                return SG_OK;
+            }
+            continue;
+        }
+        // the true total of an overflowing stage is in need_*; stages after
+        // it ran on clamped data and are re-measured by the next attempt
+        if (h.status & ST_CAP_P1) g->cap_P1 = grow(h.need_P1);
+        else if (h.status & ST_CAP_C2) g->cap_C2 = grow(h.need_C2);
+        else if (h.status & ST_CAP_P2) g->cap_P2 = grow(h.need_P2);
+    }
+    sg_grid_destroy(g);
+    return SG_ERR_CAPACITY;
+}
+
+static sg_status rebuild_impl(sg_ctx* ctx, sg_scene* scene, sg_grid* g, void* stream, bool fixed) {
+    if (!ctx || !scene || !g) return SG_ERR_INVALID_ARG;
+    if (scene->n != g->n_tris) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    g->scene = scene;
+    g->stream = (hipStream_t)stream;
+    enqueue_build(g, fixed, g->stream);
+    HIPCHK(hipGetLastError());
+    return SG_OK;
+}
+
+sg_status sg_rebuild(sg_ctx* ctx, sg_scene* scene, sg_grid* g, void* stream) {
+    return rebuild_impl(ctx, scene, g, stream, true);
+}
+
+sg_status sg_rebuild_free_res(sg_ctx* ctx, sg_scene* scene, sg_grid* g, void* stream) {
+    return rebuild_impl(ctx, scene, g, stream, false);
+}
+
+sg_status sg_grid_get_info(sg_grid* g, sg_grid_info* out) {
+    if (!g || !out) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(g->ctx->device));
+    GridHeader h;
+    sg_status r = read_header(g, &h);
+    if (r != SG_OK) return r;
+    memset(out, 0, sizeof(*out));
+    out->kind = g->kind;
+    out->status = status_from_bits(h.status);
+    for (int i = 0; i < 3; ++i) {
+        out->res[i] = h.res[i];
+        out->bmin[i] = h.bmin[i];
+        out->bmax[i] = h.bmax[i];
+        out->cell_size[i] = h.cs[i];
+        out->cell_size_rcp[i] = h.cell_rcp[i];
+        out->build_rcp[i] = h.rcp_build[i];
+    }
+    out->n_tris = g->n_tris;
+    out->C1 = h.C1;
+    out->P1 = h.P1;
+    out->P1_valid = h.P1_valid;
+    out->C2 = h.C2;
+    out->P2 = h.P2;
+    out->P2_valid = h.P2_valid;
+    out->cap_C1 = g->cap_C1;
+    out->cap_P1 = g->cap_P1;
+    out->cap_C2 = g->cap_C2;
+    out->cap_P2 = g->cap_P2;
+    out->device_bytes = g->pool_bytes;
+    return SG_OK;
+}
+
+sg_status sg_grid_download(sg_grid* g, int which, void* host, size_t bytes) {
+    if (!g || !host) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(g->ctx->device));
+    GridHeader h;
+    sg_status r = read_header(g, &h);
+    if (r != SG_OK) return r;
+    const bool two = g->kind == SG_GRID_TWO_LEVEL;
+    const int f1 = g->passes1 & 1, f2 = g->passes2 & 1;
+    const void* src = nullptr;
+    size_t n = 0;
+    switch (which) {
+        case SG_ARRAY_TOP_KEYS: src = g->k1[f1]; n = (size_t)h.P1 * 4; break;
+        case SG_ARRAY_TOP_VALS: src = g->v1[f1]; n = (size_t)h.P1 * 4; break;
+        case SG_ARRAY_TOP_RANGES: src = g->ranges1; n = (size_t)h.C1 * 8; break;
+        case SG_ARRAY_TOP_CELLS:
+            if (!two) return SG_ERR_INVALID_ARG;
+            src = g->cells1; n = (size_t)h.C1 * 8; break;
+        case SG_ARRAY_LEAF_KEYS:
+            if (!two) return SG_ERR_INVALID_ARG;
+            src = g->k2[f2]; n = (size_t)h.P2 * 4; break;
+        case SG_ARRAY_LEAF_VALS:
+            if (!two) return SG_ERR_INVALID_ARG;
+            src = g->v2[f2]; n = (size_t)h.P2 * 4; break;
+        case SG_ARRAY_LEAVES:
+            if (!two) return SG_ERR_INVALID_ARG;
+            src = g->leaves; n = (size_t)h.C2 * 8; break;
+        case SG_ARRAY_INDICES:
+            src = two ? g->v2[f2] : g->v1[f1];
+            n = (size_t)(two ? h.P2 : h.P1) * 4;
+            break;
+        case SG_ARRAY_TRI_COUNTS: src = g->dbg_tri; n = (size_t)g->n_tris * 4; break;
+        case SG_ARRAY_LEAF_COUNTS: src = g->dbg_leafc; n = (size_t)h.C1 * 4; break;
+        case SG_ARRAY_PAIR_COUNTS: src = g->dbg_pair; n = (size_t)h.P1 * 4; break;
+        default: return SG_ERR_INVALID_ARG;
+    }
+    if (!src) return SG_ERR_INVALID_ARG;
+    if (bytes < n) return SG_ERR_INVALID_ARG;
+    if (n) HIPCHK(hipMemcpy(host, src, n, hipMemcpyDeviceToHost));
+    return SG_OK;
+}
+
+sg_status sg_grid_destroy(sg_grid* g) {
+    if (!g) return SG_ERR_INVALID_ARG;
+    hipSetDevice(g->ctx->device);
+    if (g->pool) {
+        hipStreamSynchronize(g->stream);
+        hipFree(g->pool);
+    }
+    delete g;
+    return SG_OK;
+}
+
+// ---------------------------------------------------------------- traversal
+static GridView grid_view(const sg_grid* g) {
+    GridView v;
+    const bool two = g->kind == SG_GRID_TWO_LEVEL;
+    const int f1 = g->passes1 & 1, f2 = g->passes2 & 1;
+    v.hdr = g->hdr;
+    v.cells = two ? g->cells1 : g->ranges1;
+    v.leaves = two ? g->leaves : nullptr;
+    v.indices = two ? g->v2[f2] : g->v1[f1];
+    v.rec = g->scene->rec;
+    v.kind = g->kind;
+    return v;
+}
+
+static RaySoA ray_soa(const sg_rays* r) {
+    RaySoA s;
+    for (int i = 0; i < 3; ++i) {
+        s.a[i] = r->o[i];
+        s.a[3 + i] = r->d[i];
+    }
+    s.a[6] = r->tmax;
+    return s;
+}
+
+sg_status sg_trace_closest(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint32_t n, const uint32_t* d_count,
+                           uint32_t* d_prim, float* d_t, uint32_t* d_stats, void* stream) {
+    if (!ctx || !g || !rays || !d_prim || !d_t) return SG_ERR_INVALID_ARG;
+    for (int i = 0; i < 3; ++i)
+        if (!rays->o[i] || !rays->d[i]) return SG_ERR_INVALID_ARG;
+    if (!rays->tmax) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    launch_trace(grid_view(g), ray_soa(rays), n, d_count, d_prim, d_t, d_stats, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return SG_OK;
+}
+
+sg_status sg_bounce(sg_ctx* ctx, sg_scene* scene, const sg_rays* rays, const uint32_t* d_prim, const float* d_t,
+                    uint32_t n, uint64_t seed, const sg_rays_out* out, uint32_t* d_src, uint32_t* d_count,
+                    void* stream) {
+    if (!ctx || !scene || !rays || !d_prim || !d_t || !out || !d_count) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    void* scratch = nullptr;
+    sg_status r = ctx_scratch(ctx, bounce_scratch_bytes(n), &scratch);
+    if (r != SG_OK) return r;
+    RaySoAOut o;
+    for (int i = 0; i < 3; ++i) {
+        o.a[i] = out->o[i];
+        o.a[3 + i] = out->d[i];
+    }
+    o.a[6] = out->tmax;
+    launch_bounce(scene_soa(scene), ray_soa(rays), d_prim, d_t, n, seed, o, d_src, d_count, scratch,
+                  (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return SG_OK;
+}
+
+// --------------------------------------------------------------- primitives
+sg_status sg_scan_u32(sg_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint32_t n, int inclusive, void* stream) {
+    if (!ctx || (!d_in && n) || (!d_out && n)) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    void* scratch = nullptr;
+    sg_status r = ctx_scratch(ctx, scan_scratch_bytes(n), &scratch);
+    if (r != SG_OK) return r;
+    launch_scan(d_in, d_out, n, inclusive != 0, scratch, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return SG_OK;
+}
+
+__global__ void k_sort_setup(uint32_t* hdr_words, uint32_t n, size_t zero_words, uint32_t* zero_base) {
+    if (blockIdx.x == 0 && threadIdx.x < 64) hdr_words[threadIdx.x] = threadIdx.x == 0 ? n : 0u;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < zero_words; i += (size_t)gridDim.x * blockDim.x)
+        zero_base[i] = 0u;
+}
+
+sg_status sg_sort_pairs(sg_ctx* ctx, uint32_t* d_keys, uint32_t* d_vals, uint32_t n, int key_bits, int radix_bits,
+                        void* stream) {
+    if (!ctx || !d_keys || !d_vals || key_bits < 1 || key_bits > 32) return SG_ERR_INVALID_ARG;
+    if (radix_bits == 0) radix_bits = 8;
+    if (radix_bits != 4 && radix_bits != 6 && radix_bits != 8) return SG_ERR_INVALID_ARG;
+    if (n >= (1u << 30)) return SG_ERR_INVALID_ARG;
+    if (n < 2) return SG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    // scratch: [64 words: n, status, tile counters][keys n][vals n][sort scratch]
+    const size_t sw = sort_scratch_words(n, key_bits, radix_bits);
+    const size_t bytes = 256 + align_up((size_t)n * 4, 256) * 2 + sw * 4;
+    void* scratch = nullptr;
+    sg_status r = ctx_scratch(ctx, bytes, &scratch);
+    if (r != SG_OK) return r;
+    uint32_t* words = (uint32_t*)scratch;
+    uint32_t* tk = (uint32_t*)((char*)scratch + 256);
+    uint32_t* tv = (uint32_t*)((char*)tk + align_up((size_t)n * 4, 256));
+    uint32_t* ss = (uint32_t*)((char*)tv + align_up((size_t)n * 4, 256));
+    hipLaunchKernelGGL(k_sort_setup, dim3(std::min<size_t>((sw + 255) / 256 + 1, 2048)), dim3(256), 0, st, words, n,
+                       sw, ss);
+    launch_sort_pairs(d_keys, d_vals, tk, tv, words, n, key_bits, radix_bits, ss, words + 8, words + 1, st);
+    if (sort_passes(key_bits, radix_bits) & 1) {
+        HIPCHK(hipMemcpyAsync(d_keys, tk, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_vals, tv, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    }
+    HIPCHK(hipGetLastError());
+    return SG_OK;
+}
+
+// ---------------------------------------------------------------- plumbing
+sg_status sg_malloc(sg_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) return SG_ERR_OUT_OF_MEMORY;
+    return SG_OK;
+}
+
+sg_status sg_free(sg_ctx* ctx, void* p) {
+    if (!ctx) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (p) HIPCHK(hipFree(p));
+    return SG_OK;
+}
+
+sg_status sg_memcpy_htod(sg_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return SG_OK;
+}
+
+sg_status sg_memcpy_dtoh(sg_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return SG_OK;
+}
+
+sg_status sg_stream_create(sg_ctx* ctx, void** out) {
+    if (!ctx || !out) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *out = s;
+    return SG_OK;
+}
+
+sg_status sg_stream_destroy(sg_ctx* ctx, void* s) {
+    if (!ctx || !s) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipStreamDestroy((hipStream_t)s));
+    return SG_OK;
+}
+
+sg_status sg_stream_sync(sg_ctx* ctx, void* s) {
+    if (!ctx) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipStreamSynchronize((hipStream_t)s));
+    return SG_OK;
+}
+
+sg_status sg_event_create(sg_ctx* ctx, void** out) {
+    if (!ctx || !out) return SG_ERR_INVALID_ARG;
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    *out = e;
+    return SG_OK;
+}
+
+sg_status sg_event_destroy(sg_ctx* ctx, void* ev) {
+    if (!ctx || !ev) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipEventDestroy((hipEvent_t)ev));
+    return SG_OK;
+}
+
+sg_status sg_event_record(sg_ctx* ctx, void* ev, void* stream) {
+    if (!ctx || !ev) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+    return SG_OK;
+}
+
+sg_status sg_event_elapsed_ms(sg_ctx* ctx, void* a, void* b, float* ms) {
+    if (!ctx || !a || !b || !ms) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipEventSynchronize((hipEvent_t)b));
+    HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+    return SG_OK;
+}
+
+}  // extern "C"

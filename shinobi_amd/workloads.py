@@ -1,0 +1,66 @@
+"""Synthetic workloads of the grid-acceleration path (scene, rays, build
+parameters), shared by the tests, the smoke check and bench.py.
+
+* ``soup10k``   -- 10,000 triangles, centroid U[0,1]^3, vertex offsets
+  U[-0.01,0.01]^3 (seed 1); 256x256 pinhole rays from (0.5,0.5,-1) toward +z,
+  60 degree vertical FOV; uniform grid, lambda 5.
+* ``heightfield2m`` -- 1024x1024-vertex height field over [0,1]^2 =
+  2,093,058 triangles, 8-octave value noise of amplitude 0.2 (seed 2);
+  1920x1080 rays from (0.5,-0.3,0.6) toward the field centre plus one
+  cosine-weighted bounce per hit (seed 102); two-level grid, lambda 5 / 6 top,
+  lambda 2 leaf, 16-reference leaf threshold.
+* ``cluster`` -- 500,000 tiny triangles in a sphere of radius 0.02 at
+  (0.5,0.5,0.5) plus 2,000 large triangles in [0,10]^3 (seed 3); two-level.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import gen_camera_rays, gen_cluster, gen_heightfield, gen_soup, heightfield_height
+from ._lib import SG_GRID_TWO_LEVEL, SG_GRID_UNIFORM
+
+
+@dataclass
+class Workload:
+    name: str
+    kind: int
+    scene: np.ndarray                 # (9, N) float32
+    rays: np.ndarray                  # (7, R) float32 primary rays
+    build: dict = field(default_factory=dict)
+    bounce_seed: int | None = None
+    # oracle keyword arguments for the same build
+    oracle: dict = field(default_factory=dict)
+
+
+def soup10k(n: int = 10_000, res: int = 256, seed: int = 1) -> Workload:
+    scene = gen_soup(seed, n, 0.01)
+    rays = gen_camera_rays((0.5, 0.5, -1.0), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0, res, res)
+    return Workload("soup10k", SG_GRID_UNIFORM, scene, rays, dict(lam=5.0),
+                    oracle=dict(lam=5.0, top_divisor=1))
+
+
+def heightfield_camera(seed: int, res_x: int, res_y: int, amplitude: float = 0.2) -> np.ndarray:
+    hc = heightfield_height(seed, 0.5, 0.5, amplitude)
+    eye = (0.5, -0.3, 0.6)
+    fwd = (0.5 - eye[0], 0.5 - eye[1], hc - eye[2])
+    return gen_camera_rays(eye, fwd, (0.0, 0.0, 1.0), 60.0, res_x, res_y)
+
+
+def heightfield2m(grid: int = 1024, res_x: int = 1920, res_y: int = 1080, seed: int = 2) -> Workload:
+    scene = gen_heightfield(seed, grid, 0.2)
+    rays = heightfield_camera(seed, res_x, res_y)
+    return Workload("heightfield2m", SG_GRID_TWO_LEVEL, scene, rays,
+                    dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16), bounce_seed=seed + 100,
+                    oracle=dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16))
+
+
+def cluster(n_small: int = 500_000, n_large: int = 2_000, res_x: int = 1280, res_y: int = 720,
+            seed: int = 3) -> Workload:
+    scene = gen_cluster(seed, n_small, n_large)
+    eye = (0.5, 0.5, -0.6)
+    rays = gen_camera_rays(eye, (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0, res_x, res_y)
+    return Workload("cluster", SG_GRID_TWO_LEVEL, scene, rays,
+                    dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16), bounce_seed=seed + 100,
+                    oracle=dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16))

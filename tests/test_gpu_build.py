@@ -1,0 +1,164 @@
+"""GPU parity of the grid builds against the oracle: every array bit-exact.
+
+Inputs are the seeded synthetic workloads; the oracle builds the same scene
+with the same parameters on the CPU.
+"""
+import numpy as np
+import pytest
+
+import shinobi_amd as sa
+from shinobi_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_build(ctx, soa9, kind, **kw):
+    scene = sa.FaceSoup(ctx, soa9)
+    B = sa.FastTwoLevelGridBuilder if kind == 1 else sa.FastSimpleGridBuilder
+    b = B(ctx, keep_debug=1, **kw)
+    b.init(scene)
+    g = b.build()
+    return scene, g
+
+
+def _assert_same_params(info, og):
+    assert info.status == 0
+    assert tuple(info.res) == tuple(og.res)
+    np.testing.assert_array_equal(info.bmin, og.bmin)
+    np.testing.assert_array_equal(info.bmax, og.bmax)
+    np.testing.assert_array_equal(info.cell_size, og.cs)
+    np.testing.assert_array_equal(info.build_rcp, og.rcp_build)
+    np.testing.assert_array_equal(info.cell_size_rcp, og.cell_rcp)
+    assert (info.C1, info.P1, info.P1_valid) == (og.C1, og.P1, og.P1_valid)
+
+
+def _assert_top_level(g, og):
+    np.testing.assert_array_equal(g.download("tri_counts"), og.tri_counts)
+    np.testing.assert_array_equal(g.download("top_keys"), og.top_keys)
+    np.testing.assert_array_equal(g.download("top_vals"), og.top_vals)
+    np.testing.assert_array_equal(g.download("top_ranges"), og.top_ranges)
+
+
+def _assert_leaf_level(g, og):
+    info = g.info()
+    assert (info.C2, info.P2, info.P2_valid) == (og.C2, og.P2, og.P2_valid)
+    np.testing.assert_array_equal(g.download("leaf_counts"), og.leaf_counts)
+    np.testing.assert_array_equal(g.download("top_cells"), og.top_cells)
+    np.testing.assert_array_equal(g.download("pair_counts"), og.pair_counts)
+    np.testing.assert_array_equal(g.download("leaf_keys"), og.leaf_keys)
+    np.testing.assert_array_equal(g.download("leaf_vals"), og.leaf_vals)
+    np.testing.assert_array_equal(g.download("leaves"), og.leaves)
+    np.testing.assert_array_equal(g.download("indices"), og.indices)
+
+
+def test_uniform_soup10k_bit_exact(ctx, oracle):
+    w = W.soup10k()
+    og = oracle.build_uniform(w.scene, **w.oracle)
+    assert og.status == 0
+    _, g = _gpu_build(ctx, w.scene, 0, **w.build)
+    _assert_same_params(g.info(), og)
+    _assert_top_level(g, og)
+    np.testing.assert_array_equal(g.download("indices"), og.indices)
+
+
+@pytest.mark.parametrize("grid", [64, 256])
+def test_twolevel_heightfield_small_bit_exact(ctx, oracle, grid):
+    w = W.heightfield2m(grid=grid, res_x=64, res_y=32)
+    og = oracle.build_twolevel(w.scene, **w.oracle)
+    assert og.status == 0
+    _, g = _gpu_build(ctx, w.scene, 1, **w.build)
+    _assert_same_params(g.info(), og)
+    _assert_top_level(g, og)
+    _assert_leaf_level(g, og)
+
+
+def test_twolevel_heightfield2m_bit_exact(ctx, oracle):
+    w = W.heightfield2m(res_x=64, res_y=32)
+    assert w.scene.shape[1] == 2_093_058
+    og = oracle.build_twolevel(w.scene, **w.oracle)
+    assert og.status == 0
+    _, g = _gpu_build(ctx, w.scene, 1, **w.build)
+    _assert_same_params(g.info(), og)
+    _assert_top_level(g, og)
+    _assert_leaf_level(g, og)
+
+
+def test_twolevel_cluster_bit_exact(ctx, oracle):
+    w = W.cluster(n_small=60_000, n_large=400, res_x=32, res_y=32)
+    og = oracle.build_twolevel(w.scene, **w.oracle)
+    _, g = _gpu_build(ctx, w.scene, 1, **w.build)
+    info = g.info()
+    assert info.status == og.status
+    _assert_top_level(g, og)
+    _assert_leaf_level(g, og)
+
+
+def test_twolevel_reference_leaf_lambda(ctx, oracle):
+    """lambda_leaf = 1.2 (TwoLevelBuildKernels.h:49) on the 10k soup"""
+    w = W.soup10k()
+    og = oracle.build_twolevel(w.scene, lam=5.0, top_divisor=6, lambda_leaf=1.2, leaf_min_refs=16)
+    _, g = _gpu_build(ctx, w.scene, 1, lam=5.0, top_divisor=6, lambda_leaf=1.2, leaf_min_refs=16)
+    _assert_same_params(g.info(), og)
+    _assert_top_level(g, og)
+    _assert_leaf_level(g, og)
+
+
+@pytest.mark.parametrize("radix_bits", [4, 6, 8])
+def test_radix_width_does_not_change_the_grid(ctx, oracle, radix_bits):
+    w = W.heightfield2m(grid=200, res_x=8, res_y=8)
+    og = oracle.build_twolevel(w.scene, **w.oracle)
+    _, g = _gpu_build(ctx, w.scene, 1, radix_bits=radix_bits, **w.build)
+    _assert_top_level(g, og)
+    _assert_leaf_level(g, og)
+
+
+def test_build_is_deterministic_and_rebuild_matches(ctx, oracle):
+    w = W.heightfield2m(grid=300, res_x=8, res_y=8)
+    scene, g = _gpu_build(ctx, w.scene, 1, **w.build)
+    first = {k: g.download(k) for k in ("top_keys", "top_vals", "leaf_keys", "leaf_vals", "leaves", "top_cells")}
+    g.rebuild()
+    for k, v in first.items():
+        np.testing.assert_array_equal(g.download(k), v)
+    # move the vertices (same topology); rebuild keeps the resolution
+    res = g.info().res
+    moved = w.scene.copy()
+    moved[2] *= np.float32(1.1)
+    moved[5] *= np.float32(1.1)
+    moved[8] *= np.float32(1.1)
+    scene.upload_vertices(moved)
+    g.rebuild()
+    og = oracle.build_twolevel(moved, res_override=res, **w.oracle)
+    info = g.info()
+    assert info.status == 0 and tuple(info.res) == tuple(og.res)
+    _assert_top_level(g, og)
+    _assert_leaf_level(g, og)
+
+
+def test_edge_scenes(ctx, oracle):
+    # one triangle
+    one = np.array([[0.], [0.], [0.], [1.], [0.], [0.2], [0.], [1.], [0.5]], np.float32)
+    for kind in (0, 1):
+        og = (oracle.build_twolevel if kind else oracle.build_uniform)(one, lam=5.0, top_divisor=6 if kind else 1)
+        _, g = _gpu_build(ctx, one, kind, lam=5.0)
+        _assert_top_level(g, og)
+        if kind:
+            _assert_leaf_level(g, og)
+    # degenerate (zero-area) triangles mixed in: plane test rejects them when they span cells
+    w = sa.gen_soup(7, 3000, 0.05)
+    w[3:6, ::7] = w[0:3, ::7]
+    w[6:9, ::7] = w[0:3, ::7] + np.float32(0.2)
+    og = oracle.build_twolevel(w, lam=5.0, top_divisor=6, lambda_leaf=2.0)
+    _, g = _gpu_build(ctx, w, 1, lam=5.0, lambda_leaf=2.0)
+    _assert_top_level(g, og)
+    _assert_leaf_level(g, og)
+
+
+def test_invalid_scenes_are_reported(ctx):
+    flat = sa.gen_soup(3, 100, 0.01)
+    flat[[2, 5, 8]] = 0.25      # zero extent in z
+    scene = sa.FaceSoup(ctx, flat)
+    b = sa.FastSimpleGridBuilder(ctx)
+    b.init(scene)
+    with pytest.raises(sa.ShinobiError) as e:
+        b.build()
+    assert e.value.status == 2
