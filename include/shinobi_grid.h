@@ -145,6 +145,13 @@ sg_status sg_build(sg_ctx *ctx, sg_scene *scene, const sg_build_params *p, void 
 sg_status sg_rebuild(sg_ctx *ctx, sg_scene *scene, sg_grid *grid, void *stream);
 /* like sg_rebuild but lets the device recompute the resolution as well */
 sg_status sg_rebuild_free_res(sg_ctx *ctx, sg_scene *scene, sg_grid *grid, void *stream);
+/* sg_rebuild that also records hipEvent_t events[i] at the phase boundaries,
+ * the analogue of the builders' phase timers (FastTwoLevelGridBuilder.h:
+ * 755-781): [0] start, [1] bounds, [2] top emission, [3] top sort, [4] top
+ * ranges, and for the two-level grid [5] leaf cells, [6] leaf emission,
+ * [7] leaf sort, [8] leaf ranges.  Null entries are skipped. */
+sg_status sg_rebuild_phased(sg_ctx *ctx, sg_scene *scene, sg_grid *grid, void *stream, void *const *events,
+                            int n_events);
 /* waits for the grid's stream, reads the device header */
 sg_status sg_grid_get_info(sg_grid *grid, sg_grid_info *out);
 sg_status sg_grid_download(sg_grid *grid, int which, void *host, size_t bytes);

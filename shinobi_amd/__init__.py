@@ -305,6 +305,13 @@ class Grid:
         f = self.ctx._l.sg_rebuild_free_res if free_resolution else self.ctx._l.sg_rebuild
         check(f(self.ctx.h, self.scene.h, self.h, self.ctx.stream), "sg_rebuild")
 
+    def rebuild_phased(self, events: list["Event"]) -> None:
+        """rebuild() recording events at the phase boundaries (see
+        sg_rebuild_phased in include/shinobi_grid.h)."""
+        arr = (C.c_void_p * len(events))(*[e.h.value for e in events])
+        check(self.ctx._l.sg_rebuild_phased(self.ctx.h, self.scene.h, self.h, self.ctx.stream, arr, len(events)),
+              "sg_rebuild_phased")
+
     def trace(self, rays: RayBatch, prim: DeviceBuffer | None = None, t: DeviceBuffer | None = None,
               stats: DeviceBuffer | None = None, d_count: C.c_void_p | None = None, n: int | None = None):
         n = rays.n if n is None else n

@@ -83,6 +83,8 @@ SIGNATURES = {
     "sg_build": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BuildParams), C.c_void_p, C.POINTER(C.c_void_p)]),
     "sg_rebuild": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "sg_rebuild_free_res": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sg_rebuild_phased": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p),
+                                    C.c_int]),
     "sg_grid_get_info": (C.c_int, [C.c_void_p, C.POINTER(GridInfo)]),
     "sg_grid_download": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
     "sg_grid_destroy": (C.c_int, [C.c_void_p]),

@@ -300,7 +300,16 @@ static sg_status grid_alloc(sg_grid* g) {
     return SG_OK;
 }
 
-static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st) {
+// Enqueue the whole build on `st`.  marks (optional, n_marks entries) are
+// hipEvent_t recorded at the phase boundaries: [0] start, [1] after
+// init+bounds, [2] top emission, [3] top sort, [4] top ranges, then for the
+// two-level grid [5] leaf cells, [6] leaf emission, [7] leaf sort, [8] leaf
+// ranges.
+static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st, void* const* marks = nullptr,
+                          int n_marks = 0) {
+    auto mark = [&](int i) {
+        if (marks && i < n_marks && marks[i]) (void)hipEventRecord((hipEvent_t)marks[i], st);
+    };
     const bool two = g->kind == SG_GRID_TWO_LEVEL;
     const TriSoA t = scene_soa(g->scene);
     const uint32_t N = g->n_tris;
@@ -334,20 +343,29 @@ static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st) {
     }
     // per-pair leaf counts are only written below P1_valid
     seg(g->dbg_pair, (size_t)g->cap_P1 * 4);
+    mark(0);
     launch_init(g->hdr, N, z, st);
     launch_bounds(t, N, g->hdr, st);
+    mark(1);
     launch_top_emit(t, N, g->hdr, a, g->k1[0], g->v1[0], g->dbg_tri, g->lb_top, st);
+    mark(2);
     launch_sort_pairs(g->k1[0], g->v1[0], g->k1[1], g->v1[1], &g->hdr->P1, g->cap_P1, g->key_bits1, g->rb, g->sort1,
                       &g->hdr->tile_ctr[3], &g->hdr->status, st);
+    mark(3);
     const int f1 = g->passes1 & 1;
     launch_ranges(g->k1[f1], &g->hdr->P1, g->cap_P1, &g->hdr->C1, g->ranges1, &g->hdr->P1_valid, st);
+    mark(4);
     if (!two) return;
     launch_leaf_cells(g->hdr, a, g->ranges1, g->cells1, g->dbg_leafc, g->lb_cells, st);
+    mark(5);
     launch_leaf_emit(t, g->hdr, a, g->k1[f1], g->v1[f1], g->cells1, g->k2[0], g->v2[0], g->dbg_pair, g->lb_leaf, st);
+    mark(6);
     launch_sort_pairs(g->k2[0], g->v2[0], g->k2[1], g->v2[1], &g->hdr->P2, g->cap_P2, g->key_bits2, g->rb, g->sort2,
                       &g->hdr->tile_ctr[3 + g->passes1], &g->hdr->status, st);
+    mark(7);
     const int f2 = g->passes2 & 1;
     launch_ranges(g->k2[f2], &g->hdr->P2, g->cap_P2, &g->hdr->C2, g->leaves, &g->hdr->P2_valid, st);
+    mark(8);
 }
 
 static sg_status read_header(sg_grid* g, GridHeader* h) {
@@ -459,13 +477,14 @@ sg_status sg_build(sg_ctx* ctx, sg_scene* scene, const sg_build_params* p, void*
     return SG_ERR_CAPACITY;
 }
 
-static sg_status rebuild_impl(sg_ctx* ctx, sg_scene* scene, sg_grid* g, void* stream, bool fixed) {
+static sg_status rebuild_impl(sg_ctx* ctx, sg_scene* scene, sg_grid* g, void* stream, bool fixed,
+                              void* const* marks = nullptr, int n_marks = 0) {
     if (!ctx || !scene || !g) return SG_ERR_INVALID_ARG;
     if (scene->n != g->n_tris) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
     g->scene = scene;
     g->stream = (hipStream_t)stream;
-    enqueue_build(g, fixed, g->stream);
+    enqueue_build(g, fixed, g->stream, marks, n_marks);
     HIPCHK(hipGetLastError());
     return SG_OK;
 }
@@ -476,6 +495,12 @@ sg_status sg_rebuild(sg_ctx* ctx, sg_scene* scene, sg_grid* g, void* stream) {
 
 sg_status sg_rebuild_free_res(sg_ctx* ctx, sg_scene* scene, sg_grid* g, void* stream) {
     return rebuild_impl(ctx, scene, g, stream, false);
+}
+
+sg_status sg_rebuild_phased(sg_ctx* ctx, sg_scene* scene, sg_grid* g, void* stream, void* const* events,
+                            int n_events) {
+    if (!events || n_events < 0) return SG_ERR_INVALID_ARG;
+    return rebuild_impl(ctx, scene, g, stream, true, events, n_events);
 }
 
 sg_status sg_grid_get_info(sg_grid* g, sg_grid_info* out) {
