@@ -1,0 +1,131 @@
+"""Pins for the oracle's traversal + Moller-Trumbore closest hit
+(src/Control/Algorithm/SimpleGridTraverser.h:274-401,
+src/Control/Algorithm/TwoLevelGridTraverser.h:445-671,
+src/Intersector/SimpleIntersector.hpp:50-89): FP64 brute force over every
+triangle is the judge, FP32 brute force with the same formula must find the
+same hit wherever the grid visits the hit triangle, and the reference's
+IntersectorTest scene pins tie-breaking."""
+import numpy as np
+import pytest
+
+import shinobi_amd as sa
+
+MISS = 0xFFFFFFFF
+
+
+@pytest.fixture(scope="module")
+def scene():
+    return sa.gen_soup(31, 3000, 0.04)
+
+
+@pytest.fixture(scope="module")
+def rays():
+    return sa.gen_random_rays(32, 6000, 0.0, 1.0)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_fp64_judge(oracle, scene, rays, kind):
+    g = (oracle.build_twolevel(scene, lambda_leaf=2.0) if kind else oracle.build_uniform(scene, top_divisor=1))
+    prim, t = g.trace(rays)
+    p64, t64 = oracle.brute_f64(scene, rays)
+    agree = np.mean(prim == p64)
+    assert agree >= 0.995, agree
+    hit = (prim == p64) & (p64 != MISS)
+    assert np.max(np.abs(t[hit] - t64[hit]) / t64[hit]) < 1e-4
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_fp32_brute_force_same_formula(oracle, scene, rays, kind):
+    """same MT arithmetic: where the grid and brute force agree on the
+    triangle, t is bit-identical; disagreements are rare (cells the plane
+    test drops, exact ties)"""
+    g = (oracle.build_twolevel(scene, lambda_leaf=2.0) if kind else oracle.build_uniform(scene, top_divisor=1))
+    prim, t = g.trace(rays)
+    pb, tb = oracle.brute_f32(scene, rays)
+    same = prim == pb
+    assert same.mean() >= 0.995
+    assert np.array_equal(t[same].view(np.uint32), tb[same].view(np.uint32))
+    # never a closer hit than brute force (the grid tests a subset)
+    hit = prim != MISS
+    assert (t[hit] >= tb[hit]).all()
+
+
+def test_tie_scene(oracle):
+    """IntersectorTest (src/Test/IntersectorTest.h:179-258): triangle 0 in the
+    plane z=-2, triangles 1..399 identical in the plane y=-4; camera at the
+    origin looking along -y.  Strict '<' keeps the first of the equal-t
+    duplicates, so the answer is 0, 1 or a miss -- exactly the FP64 judge."""
+    n = 400
+    soa = np.zeros((9, n), np.float32)
+    soa[:, 0] = [-3, -10, -2, 3, -10, -2, 0, 10, -2]
+    for i in range(1, n):
+        soa[:, i] = [-6, -4, -6, 6, -4, -6, 0, -4, 6]
+    r = sa.gen_camera_rays((0, 0, 0), (0, -1, 0), (0, 0, 1), 66.0, 64, 64)
+    p64, _ = oracle.brute_f64(soa, r)
+    for kind in (0, 1):
+        g = oracle.build_twolevel(soa) if kind else oracle.build_uniform(soa, top_divisor=1)
+        prim, _ = g.trace(r)
+        assert set(np.unique(prim)) == {0, 1, MISS}
+        assert np.mean(prim == p64) >= 0.999
+
+
+def test_axis_parallel_inside_and_short_rays(oracle, scene):
+    rng = np.random.default_rng(4)
+    n = 4000
+    r = np.zeros((7, n), np.float32)
+    r[0:3] = rng.uniform(-0.1, 1.1, (3, n))
+    ax = rng.integers(0, 3, n)
+    r[3 + ax, np.arange(n)] = rng.choice([-1.0, 1.0], n)
+    r[6] = np.float32(3.402823466e38)
+    r[6, :400] = 0.05
+    r[6, 400:420] = 0.0
+    pb, tb = oracle.brute_f32(scene, r)
+    for kind in (0, 1):
+        g = oracle.build_twolevel(scene, lambda_leaf=2.0) if kind else oracle.build_uniform(scene, top_divisor=1)
+        prim, t = g.trace(r)
+        assert (prim[400:420] == MISS).all()
+        assert (t[:400][prim[:400] != MISS] < 0.05).all()
+        assert np.mean(prim == pb) >= 0.995
+
+
+def test_non_finite_rays_terminate(oracle, scene):
+    """NaN / inf components end as misses instead of looping (step cap)"""
+    r = np.zeros((7, 6), np.float32)
+    r[0:3] = 0.5
+    r[3:6, 0] = np.nan
+    r[3:6, 1] = [np.inf, 0, 0]
+    r[0:3, 2] = [1e30, 0.5, 0.5]
+    r[3:6, 2] = [-1, 0, 0]
+    r[3:6, 3] = [0, 0, 0]
+    r[3:6, 4] = [1, 1e-30, 0]
+    r[3:6, 5] = [0.3, 0.4, 0.5]
+    r[6] = np.float32(3.402823466e38)
+    for kind in (0, 1):
+        g = oracle.build_twolevel(scene) if kind else oracle.build_uniform(scene, top_divisor=1)
+        prim, t = g.trace(r)      # returns
+        assert prim.shape == (6,)
+
+
+def test_bounce_rays(oracle, scene):
+    """one cosine-weighted bounce per hit: compacted in primary order, unit
+    directions in the hemisphere facing away from the incoming ray, origin
+    0.001 off the surface"""
+    r = sa.gen_random_rays(40, 5000, 0.0, 1.0)
+    g = oracle.build_uniform(scene, top_divisor=1)
+    prim, t = g.trace(r)
+    b, src = oracle.bounce(scene, r, prim, t, 7)
+    hit = np.flatnonzero(prim != MISS)
+    assert np.array_equal(src, hit)
+    d = b[3:6].astype(np.float64)
+    assert np.allclose(np.linalg.norm(d, axis=0), 1.0, atol=1e-5)
+    v = scene.astype(np.float64).reshape(3, 3, -1)[:, :, prim[hit]]
+    n = np.cross((v[1] - v[0]).T, (v[2] - v[0]).T).T
+    n /= np.linalg.norm(n, axis=0)
+    din = r[3:6, hit].astype(np.float64)
+    n *= -np.sign(np.sum(n * din, axis=0))
+    assert (np.sum(n * d, axis=0) >= -1e-5).all()
+    hitp = r[0:3, hit].astype(np.float64) + din * t[hit]
+    off = np.sum((b[0:3].astype(np.float64) - hitp) * n, axis=0)
+    assert np.allclose(off, 0.001, atol=2e-5)
+    # cosine weighting: E[cos theta] = 2/3
+    assert abs(np.mean(np.sum(n * d, axis=0)) - 2.0 / 3.0) < 0.03
