@@ -22,8 +22,9 @@ struct sg_ctx {
 struct sg_scene {
     sg_ctx* ctx;
     uint32_t n;
-    float* soa = nullptr;     // 9*n floats
-    float4* rec = nullptr;    // 3*n float4 traversal records
+    size_t stride = 0;        // floats between the SoA arrays (multiple of 64: 256-byte aligned arrays)
+    float* soa = nullptr;     // 9*stride floats
+    float4* rec = nullptr;    // 4*n float4 per-triangle records
 };
 
 struct sg_grid {
@@ -43,8 +44,8 @@ struct sg_grid {
     uint32_t *k2[2] = {nullptr, nullptr}, *v2[2] = {nullptr, nullptr};
     uint32_t* leaves = nullptr;
     uint32_t *dbg_tri = nullptr, *dbg_leafc = nullptr, *dbg_pair = nullptr;
-    uint64_t *lb_top = nullptr, *lb_cells = nullptr, *lb_leaf = nullptr;
-    size_t lb_words = 0;
+    uint32_t *sums_top = nullptr, *sums_cells = nullptr, *sums_leaf = nullptr;   // per-tile sums
+    uint32_t* bpart = nullptr;                                                   // bounds partials
     uint32_t *sort1 = nullptr, *sort2 = nullptr;
     size_t sort1_words = 0, sort2_words = 0;
     int key_bits1 = 1, key_bits2 = 1, passes1 = 1, passes2 = 1, rb = 8;
@@ -67,7 +68,6 @@ static sg_status status_from_bits(uint32_t b) {
     if (b & ST_INVALID) return SG_ERR_INVALID_SCENE;
     if (b & ST_TOO_MANY_CELLS) return SG_ERR_TOO_MANY_CELLS;
     if (b & (ST_CAP_P1 | ST_CAP_C1 | ST_CAP_C2 | ST_CAP_P2)) return SG_ERR_CAPACITY;
-    if (b & ST_SPIN) return SG_ERR_TIMEOUT;
     if (b & ST_NEG_EXTENT) return SG_ERR_NEG_EXTENT;
     if (b & ST_LEAF_RES) return SG_ERR_LEAF_RES;
     return SG_OK;
@@ -160,7 +160,8 @@ static sg_status ctx_scratch(sg_ctx* ctx, size_t bytes, void** out) {
 // ------------------------------------------------------------------ scenes
 static TriSoA scene_soa(const sg_scene* s) {
     TriSoA t;
-    for (int i = 0; i < 9; ++i) t.v[i] = s->soa + (size_t)i * s->n;
+    for (int i = 0; i < 9; ++i) t.v[i] = s->soa + (size_t)i * s->stride;
+    t.rec = s->rec;
     return t;
 }
 
@@ -168,10 +169,10 @@ static sg_status scene_upload(sg_scene* s, const float* const soa9[9], int on_de
     const size_t b = (size_t)s->n * sizeof(float);
     for (int i = 0; i < 9; ++i) {
         if (!soa9[i]) return SG_ERR_INVALID_ARG;
-        HIPCHK(hipMemcpyAsync(s->soa + (size_t)i * s->n, soa9[i], b,
+        HIPCHK(hipMemcpyAsync(s->soa + (size_t)i * s->stride, soa9[i], b,
                               on_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
     }
-    launch_prep_tris(scene_soa(s), s->n, s->rec, st);
+    launch_prep_tris(scene_soa(s), s->n, st);
     HIPCHK(hipGetLastError());
     if (!on_dev) HIPCHK(hipStreamSynchronize(st));
     return SG_OK;
@@ -184,8 +185,9 @@ sg_status sg_scene_create(sg_ctx* ctx, const float* const soa9[9], uint32_t n, i
     sg_scene* s = new sg_scene();
     s->ctx = ctx;
     s->n = n;
-    if (hipMalloc(&s->soa, (size_t)9 * n * sizeof(float)) != hipSuccess ||
-        hipMalloc(&s->rec, (size_t)3 * n * sizeof(float4)) != hipSuccess) {
+    s->stride = ((size_t)n + 63) & ~(size_t)63;
+    if (hipMalloc(&s->soa, (size_t)9 * s->stride * sizeof(float)) != hipSuccess ||
+        hipMalloc(&s->rec, (size_t)4 * n * sizeof(float4)) != hipSuccess) {
         if (s->soa) hipFree(s->soa);
         delete s;
         return SG_ERR_OUT_OF_MEMORY;
@@ -220,7 +222,7 @@ uint32_t sg_scene_size(const sg_scene* s) { return s ? s->n : 0u; }
 
 sg_status sg_scene_device_soa(const sg_scene* s, const float* out9[9]) {
     if (!s || !out9) return SG_ERR_INVALID_ARG;
-    for (int i = 0; i < 9; ++i) out9[i] = s->soa + (size_t)i * s->n;
+    for (int i = 0; i < 9; ++i) out9[i] = s->soa + (size_t)i * s->stride;
     return SG_OK;
 }
 
@@ -259,14 +261,10 @@ static size_t grid_layout(sg_grid* g, char* pool) {
     } else {
         g->dbg_tri = g->dbg_leafc = g->dbg_pair = nullptr;
     }
-    const size_t t_top = ((size_t)N + kBlock - 1) / kBlock;
-    const size_t t_cells = ((size_t)g->cap_C1 + kBlock - 1) / kBlock;
-    const size_t t_leaf = ((size_t)g->cap_P1 + kBlock - 1) / kBlock;
-    g->lb_words = align_up(t_top, 2) + (two ? align_up(t_cells, 2) + align_up(t_leaf, 2) : 0);
-    char* lb = take(g->lb_words * 8);
-    g->lb_top = (uint64_t*)lb;
-    g->lb_cells = lb ? g->lb_top + align_up(t_top, 2) : nullptr;
-    g->lb_leaf = lb ? g->lb_cells + align_up(t_cells, 2) : nullptr;
+    g->bpart = (uint32_t*)take(bounds_blocks(N) * 6 * 4 + 16);
+    g->sums_top = (uint32_t*)take(emit_tiles(N) * 4 + 16);
+    g->sums_cells = two ? (uint32_t*)take(cell_tiles(g->cap_C1) * 4 + 16) : nullptr;
+    g->sums_leaf = two ? (uint32_t*)take((2 * leaf_tiles(g->cap_P1) + 2) * 4 + 16) : nullptr;
     g->key_bits1 = bit_length(g->cap_C1);
     g->passes1 = sort_passes(g->key_bits1, g->rb);
     g->sort1_words = sort_scratch_words(g->cap_P1, g->key_bits1, g->rb);
@@ -334,37 +332,35 @@ static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st, void* cons
             ++z.count;
         }
     };
-    seg(g->lb_top, g->lb_words * 8);
-    seg(g->sort1, g->sort1_words * 4);
     seg(g->ranges1, (size_t)g->cap_C1 * 8);
     if (two) {
-        seg(g->sort2, g->sort2_words * 4);
         seg(g->leaves, (size_t)g->cap_C2 * 8);
     }
     // per-pair leaf counts are only written below P1_valid
     seg(g->dbg_pair, (size_t)g->cap_P1 * 4);
     mark(0);
     launch_init(g->hdr, N, z, st);
-    launch_bounds(t, N, g->hdr, st);
+    launch_bounds(t, N, g->bpart, st);
+    launch_params(g->hdr, a, g->bpart, N, true, st);
     mark(1);
-    launch_top_emit(t, N, g->hdr, a, g->k1[0], g->v1[0], g->dbg_tri, g->lb_top, st);
+    launch_top_emit(t, N, g->hdr, a, g->k1[0], g->v1[0], g->dbg_tri, g->sums_top, st);
     mark(2);
     launch_sort_pairs(g->k1[0], g->v1[0], g->k1[1], g->v1[1], &g->hdr->P1, g->cap_P1, g->key_bits1, g->rb, g->sort1,
-                      &g->hdr->tile_ctr[3], &g->hdr->status, st);
+                      st);
     mark(3);
     const int f1 = g->passes1 & 1;
     launch_ranges(g->k1[f1], &g->hdr->P1, g->cap_P1, &g->hdr->C1, g->ranges1, &g->hdr->P1_valid, st);
     mark(4);
     if (!two) return;
-    launch_leaf_cells(g->hdr, a, g->ranges1, g->cells1, g->dbg_leafc, g->lb_cells, st);
+    launch_leaf_cells(g->hdr, a, g->ranges1, g->cells1, g->dbg_leafc, g->sums_cells, st);
     mark(5);
-    launch_leaf_emit(t, g->hdr, a, g->k1[f1], g->v1[f1], g->cells1, g->k2[0], g->v2[0], g->dbg_pair, g->lb_leaf, st);
+    launch_leaf_emit(t, g->hdr, a, g->k1[f1], g->v1[f1], g->ranges1, g->cells1, g->k2[0], g->v2[0], g->dbg_pair,
+                     g->sums_leaf, st);
     mark(6);
-    launch_sort_pairs(g->k2[0], g->v2[0], g->k2[1], g->v2[1], &g->hdr->P2, g->cap_P2, g->key_bits2, g->rb, g->sort2,
-                      &g->hdr->tile_ctr[3 + g->passes1], &g->hdr->status, st);
+    launch_leaf_sort_fixup(g->k2[0], g->v2[0], g->k2[1], g->v2[1], g->hdr, g->cap_P2, g->key_bits2, g->rb, g->sort2,
+                           st);
     mark(7);
-    const int f2 = g->passes2 & 1;
-    launch_ranges(g->k2[f2], &g->hdr->P2, g->cap_P2, &g->hdr->C2, g->leaves, &g->hdr->P2_valid, st);
+    launch_ranges(g->k2[0], &g->hdr->P2, g->cap_P2, &g->hdr->C2, g->leaves, &g->hdr->P2_valid, st);
     mark(8);
 }
 
@@ -405,7 +401,8 @@ sg_status sg_build(sg_ctx* ctx, sg_scene* scene, const sg_build_params* p, void*
         ZeroSegs z;
         z.count = 0;
         launch_init(g->hdr, g->n_tris, z, st);
-        launch_bounds(scene_soa(scene), g->n_tris, g->hdr, st);
+        launch_bounds(scene_soa(scene), g->n_tris, g->bpart, st);
+        launch_params(g->hdr, BuildArgs(), g->bpart, g->n_tris, false, st);
         GridHeader h;
         r = read_header(g, &h);
         if (r != SG_OK) {
@@ -542,7 +539,7 @@ sg_status sg_grid_download(sg_grid* g, int which, void* host, size_t bytes) {
     sg_status r = read_header(g, &h);
     if (r != SG_OK) return r;
     const bool two = g->kind == SG_GRID_TWO_LEVEL;
-    const int f1 = g->passes1 & 1, f2 = g->passes2 & 1;
+    const int f1 = g->passes1 & 1, f2 = 0;   // leaf pairs always end in buffer 0
     const void* src = nullptr;
     size_t n = 0;
     switch (which) {
@@ -591,7 +588,7 @@ sg_status sg_grid_destroy(sg_grid* g) {
 static GridView grid_view(const sg_grid* g) {
     GridView v;
     const bool two = g->kind == SG_GRID_TWO_LEVEL;
-    const int f1 = g->passes1 & 1, f2 = g->passes2 & 1;
+    const int f1 = g->passes1 & 1, f2 = 0;   // leaf pairs always end in buffer 0
     v.hdr = g->hdr;
     v.cells = two ? g->cells1 : g->ranges1;
     v.leaves = two ? g->leaves : nullptr;
@@ -655,10 +652,8 @@ sg_status sg_scan_u32(sg_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint32
     return SG_OK;
 }
 
-__global__ void k_sort_setup(uint32_t* hdr_words, uint32_t n, size_t zero_words, uint32_t* zero_base) {
-    if (blockIdx.x == 0 && threadIdx.x < 64) hdr_words[threadIdx.x] = threadIdx.x == 0 ? n : 0u;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < zero_words; i += (size_t)gridDim.x * blockDim.x)
-        zero_base[i] = 0u;
+__global__ void k_sort_setup(uint32_t* hdr_words, uint32_t n) {
+    if (threadIdx.x < 64) hdr_words[threadIdx.x] = threadIdx.x == 0 ? n : 0u;
 }
 
 sg_status sg_sort_pairs(sg_ctx* ctx, uint32_t* d_keys, uint32_t* d_vals, uint32_t n, int key_bits, int radix_bits,
@@ -670,7 +665,7 @@ sg_status sg_sort_pairs(sg_ctx* ctx, uint32_t* d_keys, uint32_t* d_vals, uint32_
     if (n < 2) return SG_OK;
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = (hipStream_t)stream;
-    // scratch: [64 words: n, status, tile counters][keys n][vals n][sort scratch]
+    // scratch: [64 words: n][keys n][vals n][sort scratch]
     const size_t sw = sort_scratch_words(n, key_bits, radix_bits);
     const size_t bytes = 256 + align_up((size_t)n * 4, 256) * 2 + sw * 4;
     void* scratch = nullptr;
@@ -680,9 +675,8 @@ sg_status sg_sort_pairs(sg_ctx* ctx, uint32_t* d_keys, uint32_t* d_vals, uint32_
     uint32_t* tk = (uint32_t*)((char*)scratch + 256);
     uint32_t* tv = (uint32_t*)((char*)tk + align_up((size_t)n * 4, 256));
     uint32_t* ss = (uint32_t*)((char*)tv + align_up((size_t)n * 4, 256));
-    hipLaunchKernelGGL(k_sort_setup, dim3(std::min<size_t>((sw + 255) / 256 + 1, 2048)), dim3(256), 0, st, words, n,
-                       sw, ss);
-    launch_sort_pairs(d_keys, d_vals, tk, tv, words, n, key_bits, radix_bits, ss, words + 8, words + 1, st);
+    hipLaunchKernelGGL(k_sort_setup, dim3(1), dim3(64), 0, st, words, n);
+    launch_sort_pairs(d_keys, d_vals, tk, tv, words, n, key_bits, radix_bits, ss, st);
     if (sort_passes(key_bits, radix_bits) & 1) {
         HIPCHK(hipMemcpyAsync(d_keys, tk, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
         HIPCHK(hipMemcpyAsync(d_vals, tv, (size_t)n * 4, hipMemcpyDeviceToDevice, st));

@@ -3,17 +3,22 @@
 // The build is device-resident: bounds, resolution, counts and totals live in
 // the GridHeader in HBM, every kernel sizes itself from it, and no phase
 // reads anything back to the host.  Counting, scanning and emission of each
-// level run as ONE single-pass kernel (per-tile block scan + decoupled
-// look-back), so every pair lands at the slot that a serial scan of the
-// counts in canonical order would give it: triangles (or sorted top pairs)
-// ascending, then the reference's z/y/x loop order.
+// level are reduce-then-scan (a count kernel leaves one sum per tile; the
+// emit kernel folds the sums of the tiles before it and rescans its own), so
+// every pair lands at the slot that a serial scan of the counts in canonical
+// order would give it: triangles (or sorted top pairs) ascending, then the
+// reference's z/y/x loop order.
 #include "sg_device.hpp"
 #include "sg_kernels.hpp"
 
 namespace sg {
 
+constexpr int kItems = 4;                      // items per thread in the emission kernels
+constexpr int kTile = kBlock * kItems;         // 1024 items per tile
+constexpr uint32_t kStage = 4096;              // top pairs a tile stages in LDS before writing them out
+
 // --------------------------------------------------------------- init
-// Zero the look-back state, cell ranges and leaves; reset the header.
+// Zero the cell ranges and leaves; reset the header.
 __global__ void k_init(GridHeader* h, uint32_t n_tris, ZeroSegs segs) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         for (int i = 0; i < 3; ++i) {
@@ -24,7 +29,7 @@ __global__ void k_init(GridHeader* h, uint32_t n_tris, ZeroSegs segs) {
         h->C1 = h->P1 = h->P1_valid = h->C2 = h->P2 = h->P2_valid = 0;
         h->need_P1 = h->need_C2 = h->need_P2 = 0;
         h->status = 0;
-        for (int i = 0; i < 32; ++i) h->tile_ctr[i] = 0;
+        h->leaf_sort = 0;
     }
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (int s = 0; s < segs.count; ++s) {
@@ -35,16 +40,54 @@ __global__ void k_init(GridHeader* h, uint32_t n_tris, ZeroSegs segs) {
     }
 }
 
+SG_D void tri_load(const TriSoA& t, uint32_t i, v3& a, v3& b, v3& c) {
+    a = mk(t.v[0][i], t.v[1][i], t.v[2][i]);
+    b = mk(t.v[3][i], t.v[4][i], t.v[5][i]);
+    c = mk(t.v[6][i], t.v[7][i], t.v[8][i]);
+}
+
 // --------------------------------------------------------------- bounds
 // Scene AABB (the host loop of FastTwoLevelGridBuilder.h:75-81) as a device
-// reduction; exact because min/max never round.
-__global__ __launch_bounds__(kBlock) void k_bounds(TriSoA t, uint32_t n, GridHeader* h) {
+// reduction; exact because min/max never round.  The SoA arrays are 16-byte
+// aligned, so each lane reads four triangles of an array with one load.  The
+// same pass writes the per-triangle records (TriSoA::rec), so they always
+// match the vertices the grid is built from.  Each block leaves its six
+// partial keys in `part`; k_params folds them (no same-address atomics).
+__global__ __launch_bounds__(kBlock) void k_bounds(TriSoA t, uint32_t n, uint32_t* __restrict__ part) {
     __shared__ uint32_t s_red[6][kBlock / 64];
     uint32_t k[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const uint32_t n4 = n / 4;
+    for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < n4; q += gridDim.x * kBlock) {
+        float4 f[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) f[c] = reinterpret_cast<const float4*>(t.v[c])[q];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            uint32_t k0 = fkey(t.v[a][i]), k1 = fkey(t.v[3 + a][i]), k2 = fkey(t.v[6 + a][i]);
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                const float4 g = f[3 * v + a];
+                const uint32_t k0 = fkey(g.x), k1 = fkey(g.y), k2 = fkey(g.z), k3 = fkey(g.w);
+                k[a] = min(k[a], min(min(k0, k1), min(k2, k3)));
+                k[3 + a] = max(k[3 + a], max(max(k0, k1), max(k2, k3)));
+            }
+        }
+        store_tri_record(t.rec, 4 * q + 0, mk(f[0].x, f[1].x, f[2].x), mk(f[3].x, f[4].x, f[5].x),
+                         mk(f[6].x, f[7].x, f[8].x));
+        store_tri_record(t.rec, 4 * q + 1, mk(f[0].y, f[1].y, f[2].y), mk(f[3].y, f[4].y, f[5].y),
+                         mk(f[6].y, f[7].y, f[8].y));
+        store_tri_record(t.rec, 4 * q + 2, mk(f[0].z, f[1].z, f[2].z), mk(f[3].z, f[4].z, f[5].z),
+                         mk(f[6].z, f[7].z, f[8].z));
+        store_tri_record(t.rec, 4 * q + 3, mk(f[0].w, f[1].w, f[2].w), mk(f[3].w, f[4].w, f[5].w),
+                         mk(f[6].w, f[7].w, f[8].w));
+    }
+    for (uint32_t i = n4 * 4 + blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        v3 v0, v1, v2;
+        tri_load(t, i, v0, v1, v2);
+        store_tri_record(t.rec, i, v0, v1, v2);
+        const float w[9] = {v0.x, v0.y, v0.z, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t k0 = fkey(w[a]), k1 = fkey(w[3 + a]), k2 = fkey(w[6 + a]);
             k[a] = min(k[a], min(k0, min(k1, k2)));
             k[3 + a] = max(k[3 + a], max(k0, max(k1, k2)));
         }
@@ -64,65 +107,80 @@ __global__ __launch_bounds__(kBlock) void k_bounds(TriSoA t, uint32_t n, GridHea
         const int a = threadIdx.x;
         uint32_t r = s_red[a][0];
         for (int w = 1; w < kBlock / 64; ++w) r = a < 3 ? min(r, s_red[a][w]) : max(r, s_red[a][w]);
-        if (a < 3) atomicMin(&h->bmin_key[a], r);
-        else atomicMax(&h->bmax_key[a - 3], r);
+        part[6 * blockIdx.x + a] = r;
     }
 }
 
-// Resolve bounds and grid parameters from the header into LDS (one thread),
-// block 0 also records them in the header for later kernels and the host.
-struct Params {
-    v3 gmin, gmax, cs, rcp, crcp, resf;
+// --------------------------------------------------------------- params
+// Resolution and cell sizes from the reduced bounds, once (one thread):
+// GridBuilder::computeResolution (GridBuilder.h:41-62), the two-level top
+// resolution (FastTwoLevelGridBuilder.h:87-102) and the cell sizes of
+// FastSimpleGridBuilder.h:114-117 / SimpleGrid.h:99-107.  One block: first
+// the bounds partials of k_bounds' nb blocks are folded into the header.
+__global__ __launch_bounds__(kBlock) void k_params(GridHeader* h, BuildArgs a, const uint32_t* __restrict__ part,
+                                                   uint32_t nb, int derive) {
+    __shared__ uint32_t s_red[6][kBlock / 64];
+    uint32_t k[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    for (uint32_t b = threadIdx.x; b < nb; b += kBlock)
+        for (int i = 0; i < 6; ++i) k[i] = i < 3 ? min(k[i], part[6 * b + i]) : max(k[i], part[6 * b + i]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        for (int i = 0; i < 6; ++i) {
+            const uint32_t o = (uint32_t)__shfl_xor(k[i], d, 64);
+            k[i] = i < 3 ? min(k[i], o) : max(k[i], o);
+        }
+    if (lane_id() == 0)
+        for (int i = 0; i < 6; ++i) s_red[i][wave_id()] = k[i];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int i = 0; i < 6; ++i) {
+        uint32_t r = s_red[i][0];
+        for (int w = 1; w < kBlock / 64; ++w) r = i < 3 ? min(r, s_red[i][w]) : max(r, s_red[i][w]);
+        if (i < 3) h->bmin_key[i] = r;
+        else h->bmax_key[i - 3] = r;
+    }
+    if (!derive) return;
+    float mn[3], mx[3], cs[3], rcp[3], crcp[3];
     int32_t res[3];
-    uint32_t rx, rxy, C;
+    for (int i = 0; i < 3; ++i) {
+        mn[i] = fkey_inv(h->bmin_key[i]);
+        mx[i] = fkey_inv(h->bmax_key[i]);
+    }
+    const bool ok = derive_params(mn, mx, h->n_tris, a.lambda, a.top_divisor, a.fixed_res, res, cs, rcp, crcp);
+    const uint64_t C = ok ? (uint64_t)res[0] * (uint64_t)res[1] * (uint64_t)res[2] : 0;
+    uint32_t bad = 0;
+    if (!ok) bad |= ST_INVALID;
+    else if (C >= (1ull << 24)) bad |= ST_TOO_MANY_CELLS;
+    else if (C > a.cap_C1) bad |= ST_CAP_C1;
+    for (int i = 0; i < 3; ++i) {
+        h->bmin[i] = mn[i];
+        h->bmax[i] = mx[i];
+        h->res[i] = ok ? res[i] : 0;
+        h->cs[i] = ok ? cs[i] : 0.f;
+        h->rcp_build[i] = ok ? rcp[i] : 0.f;
+        h->cell_rcp[i] = ok ? crcp[i] : 0.f;
+    }
+    h->C1 = bad ? 0u : (uint32_t)C;
+    if (bad) h->status |= bad;
+}
+
+// grid parameters as every emission lane needs them (uniform loads)
+struct Params {
+    v3 gmin, cs, rcp, resf;
+    uint32_t rx, rxy;
     bool ok;
 };
 
-SG_D void load_params(GridHeader* h, const BuildArgs& a, Params* p) {
-    if (threadIdx.x == 0) {
-        float mn[3], mx[3], cs[3], rcp[3], crcp[3];
-        int32_t res[3];
-        for (int i = 0; i < 3; ++i) {
-            mn[i] = fkey_inv(__hip_atomic_load(&h->bmin_key[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            mx[i] = fkey_inv(__hip_atomic_load(&h->bmax_key[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        }
-        bool ok = derive_params(mn, mx, h->n_tris, a.lambda, a.top_divisor, a.fixed_res, res, cs, rcp, crcp);
-        uint64_t C = ok ? (uint64_t)res[0] * (uint64_t)res[1] * (uint64_t)res[2] : 0;
-        p->gmin = mk(mn[0], mn[1], mn[2]);
-        p->gmax = mk(mx[0], mx[1], mx[2]);
-        p->cs = mk(cs[0], cs[1], cs[2]);
-        p->rcp = mk(rcp[0], rcp[1], rcp[2]);
-        p->crcp = mk(crcp[0], crcp[1], crcp[2]);
-        for (int i = 0; i < 3; ++i) p->res[i] = res[i];
-        p->resf = mk((float)res[0], (float)res[1], (float)res[2]);
-        p->rx = (uint32_t)res[0];
-        p->rxy = f2u(p->resf.x * p->resf.y);
-        p->C = (uint32_t)C;
-        uint32_t bad = 0;
-        if (!ok) bad |= ST_INVALID;
-        else if (C >= (1ull << 24)) bad |= ST_TOO_MANY_CELLS;
-        else if (C > a.cap_C1) bad |= ST_CAP_C1;
-        p->ok = bad == 0;
-        if (blockIdx.x == 0) {
-            for (int i = 0; i < 3; ++i) {
-                h->bmin[i] = mn[i];
-                h->bmax[i] = mx[i];
-                h->res[i] = ok ? res[i] : 0;
-                h->cs[i] = cs[i];
-                h->rcp_build[i] = rcp[i];
-                h->cell_rcp[i] = crcp[i];
-            }
-            h->C1 = bad ? 0u : (uint32_t)C;
-            if (bad) atomicOr(&h->status, bad);
-        }
-    }
-    __syncthreads();
-}
-
-SG_D void tri_load(const TriSoA& t, uint32_t i, v3& a, v3& b, v3& c) {
-    a = mk(t.v[0][i], t.v[1][i], t.v[2][i]);
-    b = mk(t.v[3][i], t.v[4][i], t.v[5][i]);
-    c = mk(t.v[6][i], t.v[7][i], t.v[8][i]);
+SG_D Params read_params(const GridHeader* h) {
+    Params p;
+    p.gmin = mk(h->bmin[0], h->bmin[1], h->bmin[2]);
+    p.cs = mk(h->cs[0], h->cs[1], h->cs[2]);
+    p.rcp = mk(h->rcp_build[0], h->rcp_build[1], h->rcp_build[2]);
+    p.resf = mk((float)h->res[0], (float)h->res[1], (float)h->res[2]);
+    p.rx = (uint32_t)h->res[0];
+    p.rxy = f2u(p.resf.x * p.resf.y);
+    p.ok = h->C1 != 0;
+    return p;
 }
 
 // BBox::empty + extend x3, src/Primitive/BBox.hpp:78-82,109-119
@@ -141,42 +199,14 @@ SG_D uint32_t top_range(const Params& p, v3 mn, v3 mx, uint32_t lo[3], uint32_t 
     return (hi[0] - lo[0] + 1u) * (hi[1] - lo[1] + 1u) * (hi[2] - lo[2] + 1u);
 }
 
-// ------------------------------------------------------ top level emission
-// countRefs + InclusiveScan + buildUnsortedGrid of
-// src/Control/Algorithm/FastBuildKernels.h:31-215 fused into one pass.  One
-// triangle per thread, one tile per block.
-__global__ __launch_bounds__(kBlock) void k_top_emit(TriSoA t, uint32_t n, GridHeader* h, BuildArgs a,
-                                                     uint32_t* __restrict__ keys,
-                                                     uint32_t* __restrict__ vals,
-                                                     uint32_t* __restrict__ dbg_counts, uint64_t* lb) {
-    __shared__ Params P;
-    __shared__ uint32_t s_scan[kBlock / 64];
-    __shared__ uint32_t s_word;
-    load_params(h, a, &P);
-    const uint32_t tile = next_tile(&h->tile_ctr[0], &s_word);
-    const uint32_t i = tile * kBlock + threadIdx.x;
-    v3 v0 = mk(0, 0, 0), v1 = v0, v2 = v0, mn, mx;
-    uint32_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    uint32_t cnt = 0;
-    if (P.ok && i < n) {
-        tri_load(t, i, v0, v1, v2);
-        tri_bounds(v0, v1, v2, mn, mx);
-        cnt = top_range(P, mn, mx, lo, hi);
-        if (dbg_counts) dbg_counts[i] = cnt;
-    }
-    uint32_t agg;
-    const uint32_t local = block_excl_scan<kBlock / 64>(cnt, &agg, s_scan);
-    const uint32_t prefix = lb_tile_prefix(lb, tile, agg, &h->status, &s_word);
-    const uint32_t total_tiles = (n + kBlock - 1) / kBlock;
-    if (tile == total_tiles - 1 && threadIdx.x == 0) {
-        uint32_t P1 = prefix + agg;
-        h->need_P1 = P1;
-        h->P1 = P1 <= a.cap_P1 ? P1 : a.cap_P1;
-        if (P1 > a.cap_P1) atomicOr(&h->status, ST_CAP_P1);
-    }
-    if (cnt == 0) return;
-    uint32_t slot = prefix + local;
-    if ((uint64_t)slot + cnt > a.cap_P1) return;   // flagged above
+// buildUnsortedGrid's per-triangle emission, src/Control/Algorithm/FastBuildKernels.h:130-212
+SG_D void top_emit_one(const Params& P, const TriSoA& t, uint32_t i, uint32_t slot, uint32_t* __restrict__ keys,
+                       uint32_t* __restrict__ vals) {
+    v3 v0, v1, v2, mn, mx;
+    uint32_t lo[3], hi[3];
+    tri_load(t, i, v0, v1, v2);
+    tri_bounds(v0, v1, v2, mn, mx);
+    top_range(P, mn, mx, lo, hi);
     if (hi[0] - lo[0] > 0u && hi[1] - lo[1] > 0u && hi[2] - lo[2] > 0u) {
         const uint32_t dummy = f2u(P.resf.x * P.resf.y * P.resf.z);
         const v3 n_ = normalize(cross(v1 - v0, v2 - v0));
@@ -193,13 +223,9 @@ __global__ __launch_bounds__(kBlock) void k_top_emit(TriSoA t, uint32_t n, GridH
                 for (uint32_t x = lo[0]; x <= hi[0]; ++x, ++slot) {
                     cc.x += P.cs.x;
                     const v3 dv = scl(n_, dot(cc - v0, n_));
-                    uint32_t key;
-                    if (fabsf(dv.x) <= half.x + kEps && fabsf(dv.y) <= half.y + kEps &&
-                        fabsf(dv.z) <= half.z + kEps)
-                        key = x + y * P.rx + z * P.rxy;
-                    else
-                        key = dummy;
-                    keys[slot] = key;
+                    const bool acc = fabsf(dv.x) <= half.x + kEps && fabsf(dv.y) <= half.y + kEps &&
+                                     fabsf(dv.z) <= half.z + kEps;
+                    keys[slot] = acc ? x + y * P.rx + z * P.rxy : dummy;
                     vals[slot] = i;
                 }
             }
@@ -211,6 +237,101 @@ __global__ __launch_bounds__(kBlock) void k_top_emit(TriSoA t, uint32_t n, GridH
                     keys[slot] = x + y * P.rx + z * P.rxy;
                     vals[slot] = i;
                 }
+    }
+}
+
+// ------------------------------------------------------ top level emission
+// countRefs + InclusiveScan + buildUnsortedGrid of
+// src/Control/Algorithm/FastBuildKernels.h:31-215 as reduce-then-scan: a
+// count kernel leaves one sum per 1024-triangle tile, the emit kernel folds
+// the sums of the tiles before it, rescans its own counts and writes every
+// triangle's pairs at its slot.  Four consecutive triangles per lane (one
+// 16-byte load per SoA array).
+SG_D void top_counts(const Params& P, const TriSoA& t, uint32_t n, uint32_t i0, uint32_t cnt[kItems]) {
+    cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
+    if (!P.ok || i0 >= n) return;
+    float f[9][kItems];
+    if (i0 + kItems <= n) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const float4 q = reinterpret_cast<const float4*>(t.v[c])[i0 / 4];
+            f[c][0] = q.x; f[c][1] = q.y; f[c][2] = q.z; f[c][3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) f[c][j] = i0 + j < n ? t.v[c][i0 + j] : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        if (i0 + j < n) {
+            v3 mn, mx;
+            uint32_t lo[3], hi[3];
+            tri_bounds(mk(f[0][j], f[1][j], f[2][j]), mk(f[3][j], f[4][j], f[5][j]), mk(f[6][j], f[7][j], f[8][j]),
+                       mn, mx);
+            cnt[j] = top_range(P, mn, mx, lo, hi);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_top_count(TriSoA t, uint32_t n, GridHeader* h,
+                                                      uint32_t* __restrict__ sums,
+                                                      uint32_t* __restrict__ dbg_counts) {
+    __shared__ uint32_t s_red[kBlock / 64];
+    const Params P = read_params(h);
+    const uint32_t i0 = blockIdx.x * kTile + threadIdx.x * kItems;
+    uint32_t cnt[kItems];
+    top_counts(P, t, n, i0, cnt);
+    if (dbg_counts)
+        for (int j = 0; j < kItems; ++j)
+            if (i0 + j < n) dbg_counts[i0 + j] = cnt[j];
+    const uint32_t s = block_sum<kBlock / 64>(cnt[0] + cnt[1] + cnt[2] + cnt[3], s_red);
+    if (threadIdx.x == 0) sums[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_top_emit(TriSoA t, uint32_t n, GridHeader* h, BuildArgs a,
+                                                     uint32_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ vals,
+                                                     const uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_scan[kBlock / 64];
+    __shared__ uint32_t s_k[kStage], s_v[kStage];
+    const Params P = read_params(h);
+    const uint32_t prefix = block_prefix_of<kBlock / 64>(sums, blockIdx.x, s_scan);
+    const uint32_t i0 = blockIdx.x * kTile + threadIdx.x * kItems;
+    uint32_t cnt[kItems];
+    top_counts(P, t, n, i0, cnt);
+    const uint32_t sum = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+    uint32_t agg;
+    const uint32_t local = block_excl_scan<kBlock / 64>(sum, &agg, s_scan);
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        const uint32_t P1 = prefix + agg;
+        h->need_P1 = P1;
+        h->P1 = P1 <= a.cap_P1 ? P1 : a.cap_P1;
+        if (P1 > a.cap_P1) atomicOr(&h->status, ST_CAP_P1);
+    }
+    if ((uint64_t)prefix + agg > a.cap_P1) return;   // flagged by the last tile
+    if (agg <= kStage) {
+        // the tile's pairs are one contiguous run: build it in LDS, then
+        // write it out with full cache lines
+        uint32_t ls = local;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            if (cnt[j]) top_emit_one(P, t, i0 + j, ls, s_k, s_v);
+            ls += cnt[j];
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < agg; j += kBlock) {
+            keys[prefix + j] = s_k[j];
+            vals[prefix + j] = s_v[j];
+        }
+        return;
+    }
+    uint32_t slot = prefix + local;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        if (cnt[j]) top_emit_one(P, t, i0 + j, slot, keys, vals);
+        slot += cnt[j];
     }
 }
 
@@ -247,42 +368,64 @@ __global__ __launch_bounds__(kBlock) void k_ranges(const uint32_t* __restrict__ 
 // ------------------------------------------------------------ leaf cells
 // countLeafLevelCells<false> + ExclusiveScan + prepareTopLevelCellRanges
 // (src/Control/Algorithm/TwoLevelBuildKernels.h:40-127,
-// src/Control/Algorithm/FastTwoLevelGridBuilder.h:256-294) in one pass over
-// the top cells.
-__global__ __launch_bounds__(kBlock) void k_leaf_cells(GridHeader* h, BuildArgs a,
-                                                       const uint32_t* __restrict__ ranges,
-                                                       uint32_t* __restrict__ cells,
-                                                       uint32_t* __restrict__ dbg_counts, uint64_t* lb) {
-    __shared__ uint32_t s_scan[kBlock / 64];
-    __shared__ uint32_t s_word;
+// src/Control/Algorithm/FastTwoLevelGridBuilder.h:256-294): count kernel +
+// write kernel over tiles of 256 top cells.
+SG_D uint32_t leaf_cell(const GridHeader* h, const BuildArgs& a, const uint32_t* __restrict__ ranges, uint32_t k,
+                        uint32_t* bits) {
+    const v3 cs = mk(h->cs[0], h->cs[1], h->cs[2]);
+    const uint2 rg = reinterpret_cast<const uint2*>(ranges)[k];
+    const uint32_t r = rg.y - rg.x;
+    uint32_t rx = 1, ry = 1, rz = 1;
+    if (r > a.leaf_min_refs) {
+        const float vol = cs.x * cs.y * cs.z;
+        const float m = cbrtpow(a.lambda_leaf * (float)r / vol);
+        const v3 rf = scl(cs, m);
+        rx = f2u(rf.x); ry = f2u(rf.y); rz = f2u(rf.z);
+    }
+    *bits = (r == 0 ? 0u : kNotEmpty) | (rx & 0xFFu) | ((ry & 0xFFu) << 8) | ((rz & 0xFFu) << 16);
+    return rx * ry * rz;   // unmasked, as TwoLevelBuildKernels.h:81
+}
+
+// every leaf resolution in [1,255] (the packed bytes hold the whole value)
+SG_D bool leaf_res_ok(uint32_t bits, uint32_t cnt) {
+    const uint32_t rx = bits & 0xFFu, ry = (bits >> 8) & 0xFFu, rz = (bits >> 16) & 0xFFu;
+    return rx >= 1 && ry >= 1 && rz >= 1 && rx * ry * rz == cnt;
+}
+
+__global__ __launch_bounds__(kBlock) void k_leaf_cells_count(GridHeader* h, BuildArgs a,
+                                                             const uint32_t* __restrict__ ranges,
+                                                             uint32_t* __restrict__ sums,
+                                                             uint32_t* __restrict__ dbg_counts) {
+    __shared__ uint32_t s_red[kBlock / 64];
     const uint32_t C1 = h->C1;
-    const uint32_t tile = next_tile(&h->tile_ctr[1], &s_word);
-    const uint32_t base = tile * kBlock;
+    const uint32_t base = blockIdx.x * kBlock;
     if (base >= C1) return;
     const uint32_t k = base + threadIdx.x;
-    const v3 cs = mk(h->cs[0], h->cs[1], h->cs[2]);
     uint32_t cnt = 0, bits = 0;
     if (k < C1) {
-        const uint32_t r = ranges[2 * (size_t)k + 1] - ranges[2 * (size_t)k];
-        uint32_t rx = 1, ry = 1, rz = 1;
-        if (r > a.leaf_min_refs) {
-            const float vol = cs.x * cs.y * cs.z;
-            const float m = cbrtpow(a.lambda_leaf * (float)r / vol);
-            const v3 rf = scl(cs, m);
-            rx = f2u(rf.x); ry = f2u(rf.y); rz = f2u(rf.z);
-        }
-        if (rx < 1 || ry < 1 || rz < 1 || rx > 255 || ry > 255 || rz > 255) atomicOr(&h->status, ST_LEAF_RES);
-        cnt = rx * ry * rz;
-        bits = (r == 0 ? 0u : kNotEmpty) | (rx & 0xFFu) | ((ry & 0xFFu) << 8) | ((rz & 0xFFu) << 16);
+        cnt = leaf_cell(h, a, ranges, k, &bits);
+        if (!leaf_res_ok(bits, cnt)) atomicOr(&h->status, ST_LEAF_RES);
         if (dbg_counts) dbg_counts[k] = cnt;
     }
+    const uint32_t s = block_sum<kBlock / 64>(cnt, s_red);
+    if (threadIdx.x == 0) sums[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_leaf_cells_write(GridHeader* h, BuildArgs a,
+                                                             const uint32_t* __restrict__ ranges,
+                                                             uint32_t* __restrict__ cells,
+                                                             const uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_scan[kBlock / 64];
+    const uint32_t C1 = h->C1;
+    const uint32_t base = blockIdx.x * kBlock;
+    if (base >= C1) return;
+    const uint32_t prefix = block_prefix_of<kBlock / 64>(sums, blockIdx.x, s_scan);
+    const uint32_t k = base + threadIdx.x;
+    uint32_t cnt = 0, bits = 0;
+    if (k < C1) cnt = leaf_cell(h, a, ranges, k, &bits);
     uint32_t agg;
     const uint32_t local = block_excl_scan<kBlock / 64>(cnt, &agg, s_scan);
-    const uint32_t prefix = lb_tile_prefix(lb, tile, agg, &h->status, &s_word);
-    if (k < C1) {
-        cells[2 * (size_t)k] = prefix + local;
-        cells[2 * (size_t)k + 1] = bits;
-    }
+    if (k < C1) reinterpret_cast<uint2*>(cells)[k] = make_uint2(prefix + local, bits);
     if (base + kBlock >= C1 && threadIdx.x == 0) {
         const uint32_t C2 = prefix + agg;
         h->need_C2 = C2;
@@ -296,29 +439,70 @@ __global__ __launch_bounds__(kBlock) void k_leaf_cells(GridHeader* h, BuildArgs 
 }
 
 // ------------------------------------------------------------- leaf pairs
-// countLeafLevelRefs + InclusiveScan + writeLeafLevelRefs
-// (src/Control/Algorithm/TwoLevelBuildKernels.h:129-455) in one pass over the
-// sorted top pairs; the slot of each leaf pair follows the sorted top-pair
-// order, then the reference's z/y/x loop order.
+// countLeafLevelRefs + InclusiveScan + writeLeafLevelRefs and the leaf-level
+// radix sort (src/Control/Algorithm/TwoLevelBuildKernels.h:129-455,
+// src/Control/Algorithm/FastTwoLevelGridBuilder.h:487-488,711-712).
+//
+// The leaf pairs of top cell k carry keys in [cells[k].x, cells[k].x + leaf
+// cells of k), a range no other top cell uses, and pairs the plane test
+// rejects carry the key C2.  The stable sort of all leaf pairs by key is
+// therefore the concatenation, over top cells in order, of the stable sort of
+// each top cell's accepted pairs, followed by every rejected pair in emission
+// order.  Leaf tiles hold whole top cells: tile t covers the sorted top pairs
+// from the first top-cell start at or after t*kLeafTile to the first one at
+// or after (t+1)*kLeafTile.  The count kernel runs the plane test and leaves
+// the tile's accepted and rejected totals; the emit kernel builds the tile's
+// pairs in LDS as (tile-local key << 12 | index of the top pair within the
+// tile), sorts them by key there and writes accepted pairs to their final
+// slots and rejected ones to the tail.  A tile too large for LDS writes
+// its pairs in emission order and raises GridHeader::leaf_sort, which enables
+// a global stable radix sort of the accepted pairs (a no-op otherwise).
+constexpr uint32_t kLeafTile = 512;                   // nominal sorted top pairs per leaf tile
+constexpr int kLeafBlock = 512;                       // threads of the leaf count / emit kernels
+constexpr uint32_t kLeafStage = 4096;                 // leaf pairs a tile sorts in LDS
+constexpr int kPayBits = 12;                          // payload: top pair index within the tile
+constexpr int kLocalBits = 16;                        // tile-local key bits the LDS sort handles
+constexpr int kDigitBits = 6;                         // widest digit of an LDS sort pass
+constexpr int kLeafRounds = kLeafStage / kLeafBlock;  // LDS sort rounds per pass
+
+struct LeafParams {
+    v3 gmin, cs;
+    uint32_t rx, rxy, C1, C2, n_tris;
+};
+
 struct LeafSetup {
     v3 v0, v1, v2, org, cres;
     int lo[3], hp[3];
     uint32_t px;
 };
 
+SG_D LeafParams read_leaf_params(const GridHeader* h) {
+    LeafParams L;
+    L.gmin = mk(h->bmin[0], h->bmin[1], h->bmin[2]);
+    L.cs = mk(h->cs[0], h->cs[1], h->cs[2]);
+    L.rx = (uint32_t)h->res[0];
+    L.rxy = (uint32_t)h->res[0] * (uint32_t)h->res[1];
+    L.C1 = h->C1;
+    L.C2 = h->C2;
+    L.n_tris = h->n_tris;
+    return L;
+}
+
+// leaf-cell box of one (top cell, triangle) pair; returns the number of leaf
+// cells it spans, -1 for a negative extent
 SG_D int leaf_range(const TriSoA& t, uint32_t key, uint32_t tri, const uint32_t* __restrict__ cells,
-                    v3 gmin, v3 cs, uint32_t rx, uint32_t rxy, LeafSetup& s) {
-    const uint32_t tx = key % rx;
-    const uint32_t ty = (key % rxy) / rx;
-    const uint32_t tz = key / rxy;
+                    const LeafParams& L, LeafSetup& s) {
+    const uint32_t tx = key % L.rx;
+    const uint32_t ty = (key % L.rxy) / L.rx;
+    const uint32_t tz = key / L.rxy;
     v3 mn, mx;
-    tri_load(t, tri, s.v0, s.v1, s.v2);
+    load_tri_record(t.rec, tri, s.v0, s.v1, s.v2);
     tri_bounds(s.v0, s.v1, s.v2, mn, mx);
-    const uint2 cell = *reinterpret_cast<const uint2*>(cells + 2 * (size_t)key);
+    const uint2 cell = reinterpret_cast<const uint2*>(cells)[key];
     s.px = cell.x;
     s.cres = mk((float)(cell.y & 0xFFu), (float)((cell.y >> 8) & 0xFFu), (float)((cell.y >> 16) & 0xFFu));
-    s.org = mk((float)tx * cs.x + gmin.x, (float)ty * cs.y + gmin.y, (float)tz * cs.z + gmin.z);
-    const v3 srcp = s.cres / cs;
+    s.org = mk((float)tx * L.cs.x + L.gmin.x, (float)ty * L.cs.y + L.gmin.y, (float)tz * L.cs.z + L.gmin.z);
+    const v3 srcp = s.cres / L.cs;
     const v3 lof = (mn - s.org) * srcp + mk(-0.001f, -0.001f, -0.001f);
     const v3 hif = (mx - s.org) * srcp + mk(1.001f, 1.001f, 1.001f);
     int q;
@@ -334,58 +518,17 @@ SG_D int leaf_range(const TriSoA& t, uint32_t key, uint32_t tri, const uint32_t*
     return ex * ey * ez;
 }
 
-__global__ __launch_bounds__(kBlock) void k_leaf_emit(TriSoA t, GridHeader* h, BuildArgs a,
-                                                      const uint32_t* __restrict__ top_keys,
-                                                      const uint32_t* __restrict__ top_vals,
-                                                      const uint32_t* __restrict__ cells,
-                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                      uint32_t* __restrict__ dbg_counts, uint64_t* lb) {
-    __shared__ uint32_t s_scan[kBlock / 64];
-    __shared__ uint32_t s_word;
-    const uint32_t P1v = h->P1_valid;
-    const uint32_t tile = next_tile(&h->tile_ctr[2], &s_word);
-    const uint32_t base = tile * kBlock;
-    if (base >= P1v) return;
-    const uint32_t i = base + threadIdx.x;
-    const v3 gmin = mk(h->bmin[0], h->bmin[1], h->bmin[2]);
-    const v3 cs = mk(h->cs[0], h->cs[1], h->cs[2]);
-    const uint32_t rx = (uint32_t)h->res[0], rxy = (uint32_t)h->res[0] * (uint32_t)h->res[1];
-    const uint32_t C2 = h->C2;
-    LeafSetup s;
-    uint32_t cnt = 0, tri = 0;
-    if (i < P1v) {
-        tri = top_vals[i];
-        // a top level that overflowed its buffers may hold unwritten slots:
-        // never follow an id outside the scene
-        const uint32_t key = top_keys[i];
-        int c = (tri < h->n_tris && key < h->C1) ? leaf_range(t, key, tri, cells, gmin, cs, rx, rxy, s) : 0;
-        if (c < 0) {
-            atomicOr(&h->status, ST_NEG_EXTENT);
-            c = 0;
-        }
-        cnt = (uint32_t)c;
-        if (dbg_counts) dbg_counts[i] = cnt;
-    }
-    uint32_t agg;
-    const uint32_t local = block_excl_scan<kBlock / 64>(cnt, &agg, s_scan);
-    const uint32_t prefix = lb_tile_prefix(lb, tile, agg, &h->status, &s_word);
-    if (base + kBlock >= P1v && threadIdx.x == 0) {
-        const uint32_t P2 = prefix + agg;
-        h->need_P2 = P2;
-        if (P2 > a.cap_P2) {
-            atomicOr(&h->status, ST_CAP_P2);
-            h->P2 = a.cap_P2;
-        } else {
-            h->P2 = P2;
-        }
-    }
-    if (cnt == 0) return;
-    uint32_t slot = prefix + local;
-    if ((uint64_t)slot + cnt > a.cap_P2) return;
+// Visit the leaf cells of a pair in the reference's z/y/x order:
+// f(key, accepted).  A triangle spanning more than one leaf cell on every axis
+// keeps a cell only if the cell centre lies within half a cell (+EPS) of the
+// triangle's plane on every axis.  Keys at or beyond C2 (a leaf level that
+// overflowed its capacity) count as rejected.
+template <class F>
+SG_D void leaf_visit(const LeafParams& L, const LeafSetup& s, F&& f) {
     const uint32_t crx = f2u(s.cres.x), crxy = f2u(s.cres.x * s.cres.y);
     if (s.hp[0] - s.lo[0] > 1 && s.hp[1] - s.lo[1] > 1 && s.hp[2] - s.lo[2] > 1) {
         const v3 n_ = normalize(cross(s.v1 - s.v0, s.v2 - s.v0));
-        const v3 scs = cs / s.cres;
+        const v3 scs = L.cs / s.cres;
         const v3 c0 = (mk((float)s.lo[0], (float)s.lo[1], (float)s.lo[2]) * scs + s.org) + scl(scs, 0.5f);
         v3 cc;
         cc.z = c0.z - scs.z;
@@ -395,27 +538,363 @@ __global__ __launch_bounds__(kBlock) void k_leaf_emit(TriSoA t, GridHeader* h, B
             for (int y = s.lo[1]; y < s.hp[1]; ++y) {
                 cc.y += scs.y;
                 cc.x = c0.x - scs.x;
-                for (int x = s.lo[0]; x < s.hp[0]; ++x, ++slot) {
+                for (int x = s.lo[0]; x < s.hp[0]; ++x) {
                     cc.x += scs.x;
                     const v3 dv = scl(n_, dot(cc - s.v0, n_));
-                    uint32_t key;
-                    if (fabsf(dv.x) <= scs.x * 0.5f + kEps && fabsf(dv.y) <= scs.y * 0.5f + kEps &&
-                        fabsf(dv.z) <= scs.z * 0.5f + kEps)
-                        key = (uint32_t)x + (uint32_t)y * crx + (uint32_t)z * crxy + s.px;
-                    else
-                        key = C2;
-                    keys[slot] = key;
-                    vals[slot] = tri;
+                    const bool acc = fabsf(dv.x) <= scs.x * 0.5f + kEps && fabsf(dv.y) <= scs.y * 0.5f + kEps &&
+                                     fabsf(dv.z) <= scs.z * 0.5f + kEps;
+                    const uint32_t key = (uint32_t)x + (uint32_t)y * crx + (uint32_t)z * crxy + s.px;
+                    f(key, acc && key < L.C2);
                 }
             }
         }
     } else {
         for (int z = s.lo[2]; z < s.hp[2]; ++z)
             for (int y = s.lo[1]; y < s.hp[1]; ++y)
-                for (int x = s.lo[0]; x < s.hp[0]; ++x, ++slot) {
-                    keys[slot] = (uint32_t)x + (uint32_t)y * crx + (uint32_t)z * crxy + s.px;
-                    vals[slot] = tri;
+                for (int x = s.lo[0]; x < s.hp[0]; ++x) {
+                    const uint32_t key = (uint32_t)x + (uint32_t)y * crx + (uint32_t)z * crxy + s.px;
+                    f(key, key < L.C2);
                 }
+    }
+}
+
+// set-up of sorted top pair i; 0 pairs for a negative extent (flagged) or a
+// top level that overflowed its buffers (never follow an id outside the
+// scene or the grid)
+SG_D uint32_t leaf_pair(const TriSoA& t, const LeafParams& L, GridHeader* h, uint32_t key, uint32_t tri,
+                        const uint32_t* __restrict__ cells, LeafSetup& s) {
+    if (tri >= L.n_tris || key >= L.C1) return 0;
+    const int c = leaf_range(t, key, tri, cells, L, s);
+    if (c < 0) {
+        atomicOr(&h->status, ST_NEG_EXTENT);
+        return 0;
+    }
+    return (uint32_t)c;
+}
+
+// first top-cell start at or after sorted top pair i (P1v when none)
+SG_D uint32_t top_cell_start(uint32_t i, uint32_t P1v, uint32_t C1, const uint32_t* __restrict__ top_keys,
+                             const uint32_t* __restrict__ ranges) {
+    if (i >= P1v) return P1v;
+    const uint32_t k = top_keys[i];
+    if (k >= C1) return i;
+    const uint2 r = reinterpret_cast<const uint2*>(ranges)[k];
+    if (r.x == i) return i;
+    return (r.y > i && r.y <= P1v) ? r.y : i;
+}
+
+SG_D void leaf_tile(const GridHeader* h, const uint32_t* __restrict__ top_keys, const uint32_t* __restrict__ ranges,
+                    uint32_t t, uint32_t* a0, uint32_t* a1) {
+    const uint32_t P1v = h->P1_valid, C1 = h->C1;
+    *a0 = top_cell_start(t * kLeafTile, P1v, C1, top_keys, ranges);
+    const uint32_t e = top_cell_start((t + 1) * kLeafTile, P1v, C1, top_keys, ranges);
+    *a1 = e > *a0 ? e : *a0;
+}
+
+// sums[t] = accepted, sums[T + t] = rejected leaf pairs of tile t (T tiles)
+__global__ __launch_bounds__(kLeafBlock) void k_leaf_count(TriSoA t, GridHeader* h,
+                                                       const uint32_t* __restrict__ top_keys,
+                                                       const uint32_t* __restrict__ top_vals,
+                                                       const uint32_t* __restrict__ ranges,
+                                                       const uint32_t* __restrict__ cells,
+                                                       uint32_t* __restrict__ sums, uint32_t T,
+                                                       uint32_t* __restrict__ dbg_counts) {
+    __shared__ uint32_t s_red[kLeafBlock / 64];
+    uint32_t a0, a1;
+    leaf_tile(h, top_keys, ranges, blockIdx.x, &a0, &a1);
+    uint32_t acc = 0, all = 0;
+    if (a0 < a1) {
+        const LeafParams L = read_leaf_params(h);
+        for (uint32_t i = a0 + threadIdx.x; i < a1; i += kLeafBlock) {
+            LeafSetup s;
+            const uint32_t c = leaf_pair(t, L, h, top_keys[i], top_vals[i], cells, s);
+            if (dbg_counts) dbg_counts[i] = c;
+            if (c) {
+                all += c;
+                leaf_visit(L, s, [&](uint32_t, bool ok) { acc += ok ? 1u : 0u; });
+            }
+        }
+    }
+    const uint32_t sa = block_sum<kLeafBlock / 64>(acc, s_red);
+    const uint32_t sl = block_sum<kLeafBlock / 64>(all, s_red);
+    if (threadIdx.x == 0) {
+        sums[blockIdx.x] = sa;
+        sums[T + blockIdx.x] = sl - sa;
+    }
+}
+
+// Exclusive scan, in place, of the accepted (sums[0..T)) and rejected
+// (sums[T..2T)) tile totals; sums[2T], sums[2T+1] receive the grand totals and
+// the header the leaf-pair counts (one block of 1024 threads).
+__global__ __launch_bounds__(1024) void k_leaf_scan(GridHeader* h, BuildArgs a, uint32_t* __restrict__ sums,
+                                                    uint32_t T) {
+    __shared__ uint32_t s_scan[16];
+    uint32_t tot[2];
+    for (int half = 0; half < 2; ++half) {
+        uint32_t* row = sums + (size_t)half * T;
+        uint32_t run = 0;
+        for (uint32_t c0 = 0; c0 < T; c0 += 1024 * 8) {
+            const uint32_t i0 = c0 + threadIdx.x * 8;
+            uint32_t v[8], sum = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                v[k] = i0 + k < T ? row[i0 + k] : 0u;
+                sum += v[k];
+            }
+            uint32_t agg;
+            uint32_t p = run + block_excl_scan<16>(sum, &agg, s_scan);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (i0 + k < T) row[i0 + k] = p;
+                p += v[k];
+            }
+            run += agg;
+        }
+        tot[half] = run;
+    }
+    if (threadIdx.x == 0) {
+        const uint64_t total = (uint64_t)tot[0] + tot[1];
+        sums[2 * (size_t)T] = tot[0];
+        sums[2 * (size_t)T + 1] = tot[1];
+        h->need_P2 = (uint32_t)min(total, (uint64_t)0xFFFFFFFFu);
+        if (total > a.cap_P2) {
+            atomicOr(&h->status, ST_CAP_P2);
+            h->P2 = a.cap_P2;
+            h->P2_valid = min(tot[0], a.cap_P2);
+        } else {
+            h->P2 = (uint32_t)total;
+            h->P2_valid = tot[0];
+        }
+    }
+}
+
+// One stable LSD pass over n <= kLeafStage words: digit (x >> shift) & (2^db - 1),
+// db <= kDigitBits, in -> out, by a block of kLeafBlock threads.  Ranks within
+// a wave come from ballots over the digit bits (computed once for the digit
+// histogram and again for the scatter, which keeps registers low); waves are
+// ordered through per-wave digit counts, rounds through running bases.
+SG_D uint64_t digit_peers(bool ok, uint32_t d, int db) {
+    uint64_t peers = __ballot(ok);
+    for (int b = 0; b < db; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+SG_D void lds_sort_pass(const uint32_t* in, uint32_t* out, uint32_t n, int shift, int db, uint32_t* s_base,
+                        uint32_t* s_wc, uint32_t* s_scan) {
+    constexpr int NW = kLeafBlock / 64;
+    constexpr int RM = 1 << kDigitBits;
+    static_assert(RM * NW <= kLeafBlock, "one zeroing store per thread");
+    const uint32_t R = 1u << db, mask = R - 1u;
+    const int w = wave_id();
+    const uint64_t lt = lanemask_lt();
+    if (threadIdx.x < RM * NW) s_wc[threadIdx.x] = 0;
+    if (threadIdx.x < RM) s_base[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t r0 = 0; r0 < n; r0 += kLeafBlock) {
+        const uint32_t i = r0 + threadIdx.x;
+        const bool ok = i < n;
+        const uint32_t d = ok ? (in[i] >> shift) & mask : 0u;
+        const uint64_t peers = digit_peers(ok, d, db);
+        if (ok && (peers & lt) == 0) atomicAdd(&s_base[d], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    {
+        const uint32_t v = threadIdx.x < R ? s_base[threadIdx.x] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<NW>(v, &tot, s_scan);
+        if (threadIdx.x < R) s_base[threadIdx.x] = ex;
+        __syncthreads();
+    }
+    for (uint32_t r0 = 0; r0 < n; r0 += kLeafBlock) {
+        const uint32_t i = r0 + threadIdx.x;
+        const bool ok = i < n;
+        const uint32_t x = ok ? in[i] : 0u;
+        const uint32_t d = (x >> shift) & mask;
+        const uint64_t peers = digit_peers(ok, d, db);
+        const uint32_t rk = (uint32_t)__popcll(peers & lt);
+        if (ok && rk == 0) s_wc[w * RM + d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (ok) {
+            uint32_t pos = s_base[d] + rk;
+            for (int v = 0; v < w; ++v) pos += s_wc[v * RM + d];
+            out[pos] = x;
+        }
+        __syncthreads();
+        if (threadIdx.x < R) {
+            uint32_t s = 0;
+#pragma unroll
+            for (int v = 0; v < NW; ++v) {
+                s += s_wc[v * RM + threadIdx.x];
+                s_wc[v * RM + threadIdx.x] = 0;
+            }
+            s_base[threadIdx.x] += s;
+        }
+        __syncthreads();
+    }
+}
+
+// Where tile b's pairs go: prefixes from k_leaf_scan, tile bounds, key base
+// and local key span; `staged` when the tile fits the LDS sort.
+struct LeafTilePlan {
+    uint32_t a0, a1, pre_acc, pre_rej, tot_acc, kbase, span;
+    bool staged;
+};
+
+SG_D bool leaf_tile_plan(const GridHeader* h, const BuildArgs& a, const LeafParams& L,
+                         const uint32_t* __restrict__ top_keys, const uint32_t* __restrict__ ranges,
+                         const uint32_t* __restrict__ cells, const uint32_t* __restrict__ sums, uint32_t T,
+                         uint32_t b, LeafTilePlan& q) {
+    const uint32_t tot_acc = sums[2 * (size_t)T], tot_rej = sums[2 * (size_t)T + 1];
+    if ((uint64_t)tot_acc + tot_rej > a.cap_P2) return false;   // flagged by k_leaf_scan
+    q.tot_acc = tot_acc;
+    q.pre_acc = sums[b];
+    q.pre_rej = sums[T + b];
+    const uint32_t n_tile = ((b + 1 < T ? sums[b + 1] : tot_acc) - q.pre_acc) +
+                            ((b + 1 < T ? sums[T + b + 1] : tot_rej) - q.pre_rej);
+    leaf_tile(h, top_keys, ranges, b, &q.a0, &q.a1);
+    if (q.a0 >= q.a1) return false;
+    // tile-local keys: [0, span) for the leaf cells of the tile's top cells
+    const uint32_t k_first = top_keys[q.a0], k_last = top_keys[q.a1 - 1];
+    q.kbase = 0;
+    q.span = 0xFFFFFFFFu;
+    if (k_first < L.C1 && k_last < L.C1) {
+        const uint2 cf = reinterpret_cast<const uint2*>(cells)[k_first];
+        const uint2 cl = reinterpret_cast<const uint2*>(cells)[k_last];
+        const uint32_t nl = (cl.y & 0xFFu) * ((cl.y >> 8) & 0xFFu) * ((cl.y >> 16) & 0xFFu);
+        q.kbase = cf.x;
+        if (cl.x >= cf.x) q.span = cl.x - cf.x + nl;
+    }
+    q.staged = n_tile <= kLeafStage && q.span < (1u << kLocalBits) && q.a1 - q.a0 <= (1u << kPayBits);
+    return true;
+}
+
+// tiles that fit the LDS sort
+__global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* h, BuildArgs a,
+                                                         const uint32_t* __restrict__ top_keys,
+                                                         const uint32_t* __restrict__ top_vals,
+                                                         const uint32_t* __restrict__ ranges,
+                                                         const uint32_t* __restrict__ cells,
+                                                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                         const uint32_t* __restrict__ sums, uint32_t T) {
+    constexpr int NW = kLeafBlock / 64;
+    __shared__ uint32_t s_scan[NW];
+    __shared__ uint32_t s_a[kLeafStage], s_b[kLeafStage];
+    __shared__ uint32_t s_base[1 << kDigitBits], s_wc[(1 << kDigitBits) * NW];
+    const LeafParams L = read_leaf_params(h);
+    LeafTilePlan q;
+    if (!leaf_tile_plan(h, a, L, top_keys, ranges, cells, sums, T, blockIdx.x, q) || !q.staged) return;
+    const uint32_t a0 = q.a0, a1 = q.a1, kbase = q.kbase, span = q.span;
+    const uint32_t cap = a.cap_P2;
+    const uint32_t rej_base = q.tot_acc + q.pre_rej;
+
+    // pairs in emission order as (local key << kPayBits | top pair); a
+    // rejected pair takes the local key `span`, after every accepted one
+    uint32_t run = 0, n_acc = 0;
+    for (uint32_t c0 = a0; c0 < a1; c0 += kLeafBlock) {
+        const uint32_t i = c0 + threadIdx.x;
+        LeafSetup s;
+        uint32_t c = 0;
+        if (i < a1) c = leaf_pair(t, L, h, top_keys[i], top_vals[i], cells, s);
+        uint32_t agg;
+        uint32_t p = run + block_excl_scan<NW>(c, &agg, s_scan);
+        const uint32_t pay = i - a0;
+        if (c)
+            leaf_visit(L, s, [&](uint32_t key, bool ok) {
+                const uint32_t lk = (ok && key >= kbase && key - kbase < span) ? key - kbase : span;
+                n_acc += lk < span ? 1u : 0u;
+                if (p < kLeafStage) s_a[p] = (lk << kPayBits) | pay;
+                ++p;
+            });
+        run += agg;
+    }
+    const uint32_t acc_t = block_sum<NW>(n_acc, s_scan);   // syncs: staging complete
+    const uint32_t n = run < kLeafStage ? run : kLeafStage;
+    const int bits = 32 - __builtin_clz(span | 1u);   // local keys 0..span
+    const int passes = (bits + kDigitBits - 1) / kDigitBits;
+    const int db = (bits + passes - 1) / passes;
+    uint32_t* src = s_a;
+    uint32_t* dst = s_b;
+    for (int k = 0, lo = 0; k < passes; ++k, lo += db) {
+        lds_sort_pass(src, dst, n, kPayBits + lo, min(db, bits - lo), s_base, s_wc, s_scan);
+        uint32_t* tmp = src;
+        src = dst;
+        dst = tmp;
+    }
+    for (uint32_t j = threadIdx.x; j < n; j += kLeafBlock) {
+        const uint32_t x = src[j];
+        const uint32_t v = top_vals[a0 + (x & ((1u << kPayBits) - 1u))];
+        uint32_t idx, key;
+        if (j < acc_t) {
+            idx = q.pre_acc + j;
+            key = kbase + (x >> kPayBits);
+        } else {
+            idx = rej_base + (j - acc_t);
+            key = L.C2;
+        }
+        if (idx < cap) {
+            keys[idx] = key;
+            vals[idx] = v;
+        }
+    }
+}
+
+// tiles too large for LDS: pairs in emission order, sorted afterwards by the
+// global pass (GridHeader::leaf_sort receives the number of accepted pairs)
+__global__ __launch_bounds__(kBlock) void k_leaf_emit_large(TriSoA t, GridHeader* h, BuildArgs a,
+                                                            const uint32_t* __restrict__ top_keys,
+                                                            const uint32_t* __restrict__ top_vals,
+                                                            const uint32_t* __restrict__ ranges,
+                                                            const uint32_t* __restrict__ cells,
+                                                            uint32_t* __restrict__ keys,
+                                                            uint32_t* __restrict__ vals,
+                                                            const uint32_t* __restrict__ sums, uint32_t T) {
+    __shared__ uint32_t s_scan[kBlock / 64];
+    const LeafParams L = read_leaf_params(h);
+    LeafTilePlan q;
+    if (!leaf_tile_plan(h, a, L, top_keys, ranges, cells, sums, T, blockIdx.x, q) || q.staged) return;
+    const uint32_t cap = a.cap_P2;
+    const uint32_t rej_base = q.tot_acc + q.pre_rej;
+    if (threadIdx.x == 0) h->leaf_sort = q.tot_acc;
+    uint32_t run_acc = 0, run_rej = 0;
+    for (uint32_t c0 = q.a0; c0 < q.a1; c0 += kBlock) {
+        const uint32_t i = c0 + threadIdx.x;
+        LeafSetup s;
+        uint32_t tri = 0, c = 0, ca = 0;
+        if (i < q.a1) {
+            tri = top_vals[i];
+            c = leaf_pair(t, L, h, top_keys[i], tri, cells, s);
+            if (c) leaf_visit(L, s, [&](uint32_t, bool ok) { ca += ok ? 1u : 0u; });
+        }
+        uint32_t agg_a, agg_r;
+        uint32_t pa = q.pre_acc + run_acc + block_excl_scan<kBlock / 64>(ca, &agg_a, s_scan);
+        uint32_t pr = rej_base + run_rej + block_excl_scan<kBlock / 64>(c - ca, &agg_r, s_scan);
+        if (c)
+            leaf_visit(L, s, [&](uint32_t key, bool ok) {
+                const uint32_t idx = ok ? pa++ : pr++;
+                if (idx < cap) {
+                    keys[idx] = ok ? key : L.C2;
+                    vals[idx] = tri;
+                }
+            });
+        run_acc += agg_a;
+        run_rej += agg_r;
+    }
+}
+
+// The global stable sort of the accepted leaf pairs works in two buffers;
+// when its pass count is odd this copies its result back to the first one.
+// n = GridHeader::leaf_sort, zero when every tile was sorted in LDS.
+__global__ __launch_bounds__(kBlock) void k_copy_pairs(const uint32_t* __restrict__ ks,
+                                                       const uint32_t* __restrict__ vs, uint32_t* __restrict__ kd,
+                                                       uint32_t* __restrict__ vd, const uint32_t* d_n, uint32_t cap) {
+    const uint32_t n = min(*d_n, cap);
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        kd[i] = ks[i];
+        vd[i] = vs[i];
     }
 }
 
@@ -428,17 +907,27 @@ void launch_init(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, hipStream
     hipLaunchKernelGGL(k_init, dim3(blocks), dim3(kBlock), 0, st, h, n_tris, segs);
 }
 
-void launch_bounds(const TriSoA& t, uint32_t n, GridHeader* h, hipStream_t st) {
-    uint32_t blocks = (n + kBlock * 8 - 1) / (kBlock * 8);
-    blocks = blocks < 1 ? 1 : (blocks > 2048 ? 2048 : blocks);
-    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kBlock), 0, st, t, n, h);
+size_t bounds_blocks(uint32_t n) {
+    const size_t b = ((size_t)n + kBlock * 4 - 1) / (kBlock * 4);
+    return b < 1 ? 1 : (b > 4096 ? 4096 : b);
+}
+
+void launch_bounds(const TriSoA& t, uint32_t n, uint32_t* part, hipStream_t st) {
+    hipLaunchKernelGGL(k_bounds, dim3((uint32_t)bounds_blocks(n)), dim3(kBlock), 0, st, t, n, part);
+}
+
+void launch_params(GridHeader* h, const BuildArgs& a, const uint32_t* part, uint32_t n, bool derive,
+                   hipStream_t st) {
+    hipLaunchKernelGGL(k_params, dim3(1), dim3(kBlock), 0, st, h, a, part, (uint32_t)bounds_blocks(n),
+                       derive ? 1 : 0);
 }
 
 void launch_top_emit(const TriSoA& t, uint32_t n, GridHeader* h, const BuildArgs& a, uint32_t* keys,
-                     uint32_t* vals, uint32_t* dbg, uint64_t* lb, hipStream_t st) {
-    const uint32_t blocks = (n + kBlock - 1) / kBlock;
+                     uint32_t* vals, uint32_t* dbg, uint32_t* sums, hipStream_t st) {
+    const uint32_t blocks = (n + kTile - 1) / kTile;
     if (blocks == 0) return;
-    hipLaunchKernelGGL(k_top_emit, dim3(blocks), dim3(kBlock), 0, st, t, n, h, a, keys, vals, dbg, lb);
+    hipLaunchKernelGGL(k_top_count, dim3(blocks), dim3(kBlock), 0, st, t, n, h, sums, dbg);
+    hipLaunchKernelGGL(k_top_emit, dim3(blocks), dim3(kBlock), 0, st, t, n, h, a, keys, vals, sums);
 }
 
 void launch_ranges(const uint32_t* keys, const uint32_t* d_n, uint32_t cap, const uint32_t* d_C,
@@ -449,19 +938,44 @@ void launch_ranges(const uint32_t* keys, const uint32_t* d_n, uint32_t cap, cons
 }
 
 void launch_leaf_cells(GridHeader* h, const BuildArgs& a, const uint32_t* ranges, uint32_t* cells,
-                       uint32_t* dbg, uint64_t* lb, hipStream_t st) {
+                       uint32_t* dbg, uint32_t* sums, hipStream_t st) {
     const uint32_t blocks = (a.cap_C1 + kBlock - 1) / kBlock;
     if (blocks == 0) return;
-    hipLaunchKernelGGL(k_leaf_cells, dim3(blocks), dim3(kBlock), 0, st, h, a, ranges, cells, dbg, lb);
+    hipLaunchKernelGGL(k_leaf_cells_count, dim3(blocks), dim3(kBlock), 0, st, h, a, ranges, sums, dbg);
+    hipLaunchKernelGGL(k_leaf_cells_write, dim3(blocks), dim3(kBlock), 0, st, h, a, ranges, cells, sums);
 }
 
 void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const uint32_t* top_keys,
-                      const uint32_t* top_vals, const uint32_t* cells, uint32_t* keys, uint32_t* vals,
-                      uint32_t* dbg, uint64_t* lb, hipStream_t st) {
-    const uint32_t blocks = (a.cap_P1 + kBlock - 1) / kBlock;
-    if (blocks == 0) return;
-    hipLaunchKernelGGL(k_leaf_emit, dim3(blocks), dim3(kBlock), 0, st, t, h, a, top_keys, top_vals, cells, keys,
-                       vals, dbg, lb);
+                      const uint32_t* top_vals, const uint32_t* ranges, const uint32_t* cells, uint32_t* keys,
+                      uint32_t* vals, uint32_t* dbg, uint32_t* sums, hipStream_t st) {
+    const uint32_t T = (uint32_t)leaf_tiles(a.cap_P1);
+    hipLaunchKernelGGL(k_leaf_count, dim3(T), dim3(kLeafBlock), 0, st, t, h, top_keys, top_vals, ranges, cells, sums, T,
+                       dbg);
+    hipLaunchKernelGGL(k_leaf_scan, dim3(1), dim3(1024), 0, st, h, a, sums, T);
+    hipLaunchKernelGGL(k_leaf_emit, dim3(T), dim3(kLeafBlock), 0, st, t, h, a, top_keys, top_vals, ranges, cells, keys,
+                       vals, sums, T);
+    hipLaunchKernelGGL(k_leaf_emit_large, dim3(T), dim3(kBlock), 0, st, t, h, a, top_keys, top_vals, ranges, cells,
+                       keys, vals, sums, T);
 }
+
+void launch_leaf_sort_fixup(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, GridHeader* h, uint32_t cap,
+                            int key_bits, int radix_bits, uint32_t* scratch, hipStream_t st) {
+    launch_sort_pairs(k0, v0, k1, v1, &h->leaf_sort, cap, key_bits, radix_bits, scratch, st);
+    if (sort_passes(key_bits, radix_bits) & 1) {
+        uint32_t blocks = (cap + kBlock * 4 - 1) / (kBlock * 4);
+        blocks = blocks < 1 ? 1 : (blocks > 2048 ? 2048 : blocks);
+        hipLaunchKernelGGL(k_copy_pairs, dim3(blocks), dim3(kBlock), 0, st, k1, v1, k0, v0, &h->leaf_sort, cap);
+    }
+}
+
+// number of per-tile sums the emission kernels need for n items
+size_t emit_tiles(uint32_t n) { return ((size_t)n + kTile - 1) / kTile; }
+size_t cell_tiles(uint32_t n) { return ((size_t)n + kBlock - 1) / kBlock; }
+// leaf tiles over n sorted top pairs (the leaf sums hold 2 * tiles + 2 words)
+size_t leaf_tiles(uint32_t n) {
+    const size_t t = ((size_t)n + kLeafTile - 1) / kLeafTile;
+    return t ? t : 1;
+}
+
 
 }  // namespace sg

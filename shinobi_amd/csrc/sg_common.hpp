@@ -30,7 +30,6 @@ enum : uint32_t {
     ST_CAP_C1 = 1u << 5,         // top cells
     ST_CAP_C2 = 1u << 6,         // leaf cells
     ST_CAP_P2 = 1u << 7,         // leaf pairs
-    ST_SPIN = 1u << 8,           // a look-back wait exceeded its bound
 };
 
 // -------------------------------------------------------------- device header
@@ -47,8 +46,7 @@ struct alignas(256) GridHeader {
     uint32_t C1, P1, P1_valid, C2, P2, P2_valid;
     uint32_t need_P1, need_C2, need_P2;  // true totals (exceed cap_* on overflow)
     uint32_t status;
-    uint32_t tile_ctr[32];
-    uint32_t pad[6];
+    uint32_t leaf_sort;  // accepted leaf pairs the global sort must order (0: all tiles sorted in LDS)
 };
 
 // Build parameters passed by value to the kernels.

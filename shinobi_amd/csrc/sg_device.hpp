@@ -1,6 +1,12 @@
-// sg_device.hpp -- wave64 / block primitives for gfx950: prefix sums,
-// ballot ranking and the decoupled look-back used by the single-pass
-// count+scan+emit kernels and by the radix sort.
+// sg_device.hpp -- wave64 / block primitives for gfx950.
+//
+// Cross-block prefix sums use reduce-then-scan: a count kernel writes one
+// sum per block and the consuming kernel folds the sums of its predecessor
+// blocks itself.  No block ever waits on another block: on MI355X the eight
+// XCDs have private L2s, so a polled flag is a round trip to the memory-side
+// cache and same-address atomics serialize (~11 ns each), which makes
+// decoupled look-back chains and atomic tile tickets the slowest part of a
+// pass.
 #pragma once
 
 #include "sg_common.hpp"
@@ -48,84 +54,26 @@ SG_D uint32_t block_excl_scan(uint32_t v, uint32_t* total, uint32_t* lds) {
     return pre + inc - v;
 }
 
-// ------------------------------------------------------------ look-back
-// Status word per tile: high 32 bits = flag, low 32 bits = value.
-constexpr uint64_t LB_AGG = 1ull << 32;
-constexpr uint64_t LB_INC = 2ull << 32;
-constexpr uint32_t kSpinLimit = 1u << 24;
-
-SG_D void lb_store(uint64_t* p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-SG_D uint64_t lb_load(const uint64_t* p) {
-    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Called by every lane of ONE wave of tile `tile` (> 0) after it published
-// its aggregate.  Returns the exclusive prefix of the tile (all lanes).
-// Waits only on tiles with smaller ids, which were handed out earlier by the
-// tile counter and therefore are resident or done: every wait ends.  A wait
-// that exceeds kSpinLimit polls sets ST_SPIN and counts the tile as zero.
-SG_D uint32_t lb_prefix(const uint64_t* st, uint32_t tile, uint32_t* status) {
-    uint32_t prefix = 0;
-    int64_t base = (int64_t)tile - 1;
-    const int l = lane_id();
-    while (base >= 0) {
-        int64_t j = base - l;
-        uint64_t s = LB_INC;  // lanes past the start count as inclusive 0
-        if (j >= 0) {
-            uint32_t spins = 0;
-            s = lb_load(st + j);
-            while ((s >> 32) == 0) {
-                if (++spins > kSpinLimit) {
-                    atomicOr(status, ST_SPIN);
-                    s = LB_INC;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                s = lb_load(st + j);
-            }
-        }
-        uint64_t incm = __ballot((s >> 32) == 2);
-        uint32_t v = (uint32_t)s;
-        if (incm) {
-            int k = __ffsll((long long)incm) - 1;   // nearest inclusive predecessor
-            prefix += wave_sum(l <= k ? v : 0u);
-            break;
-        }
-        prefix += wave_sum(v);
-        base -= 64;
-    }
-    return prefix;
-}
-
-// Block-level wrapper: publish aggregate, look back with wave 0, publish the
-// inclusive value, broadcast the exclusive prefix through LDS.
-SG_D uint32_t lb_tile_prefix(uint64_t* st, uint32_t tile, uint32_t agg, uint32_t* status,
-                             uint32_t* lds_word) {
-    if (threadIdx.x < 64) {
-        uint32_t pre = 0;
-        if (tile == 0) {
-            if (threadIdx.x == 0) lb_store(st, LB_INC | (uint64_t)agg);
-        } else {
-            if (threadIdx.x == 0) lb_store(st + tile, LB_AGG | (uint64_t)agg);
-            pre = lb_prefix(st, tile, status);
-            if (threadIdx.x == 0) lb_store(st + tile, LB_INC | (uint64_t)(uint32_t)(pre + agg));
-        }
-        if (threadIdx.x == 0) *lds_word = pre;
-    }
+template <int NW>
+SG_D uint32_t block_sum(uint32_t v, uint32_t* lds) {
+    const uint32_t s = wave_sum(v);
+    if (lane_id() == 0) lds[wave_id()] = s;
     __syncthreads();
-    return *lds_word;
+    uint32_t tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) tot += lds[i];
+    __syncthreads();
+    return tot;
 }
 
-// dynamic tile id: tiles are handed out in launch order of their blocks'
-// first instruction, which makes look-back waits safe.
-SG_D uint32_t next_tile(uint32_t* ctr, uint32_t* lds_word) {
-    if (threadIdx.x == 0) *lds_word = atomicAdd(ctr, 1u);
-    __syncthreads();
-    uint32_t t = *lds_word;
-    __syncthreads();
-    return t;
+// sum of sums[0..b) computed by the whole block (the "scan" half of
+// reduce-then-scan, done redundantly by every consumer block; the sums of a
+// pass are a few KB and stay in L2)
+template <int NW>
+SG_D uint32_t block_prefix_of(const uint32_t* __restrict__ sums, uint32_t b, uint32_t* lds) {
+    uint32_t acc = 0;
+    for (uint32_t j = threadIdx.x; j < b; j += NW * 64) acc += sums[j];
+    return block_sum<NW>(acc, lds);
 }
 
 }  // namespace sg

@@ -5,10 +5,33 @@
 
 namespace sg {
 
-// nine SoA arrays: v0.xyz, v1.xyz, v2.xyz (device pointers)
+// nine SoA arrays: v0.xyz, v1.xyz, v2.xyz (device pointers), plus the
+// per-triangle 64-byte records (4 x float4) that the bounds pass writes:
+//   {v0.x, v0.y, v0.z, e1.x} {e1.y, e1.z, e2.x, e2.y} {e2.z, v1.x, v1.y, v1.z} {v2.x, v2.y, v2.z, 0}
+// with e1 = v1 - v0, e2 = v2 - v0.  Traversal reads the first 36 bytes, the
+// leaf level of the build gathers v0/v1/v2 from one aligned record instead
+// of nine SoA arrays.
 struct TriSoA {
     const float* v[9];
+    float4* rec;
 };
+
+SG_D void store_tri_record(float4* __restrict__ rec, uint32_t i, const v3& v0, const v3& v1, const v3& v2) {
+    const v3 e1 = v1 - v0, e2 = v2 - v0;
+    float4* r = rec + 4 * (size_t)i;
+    r[0] = make_float4(v0.x, v0.y, v0.z, e1.x);
+    r[1] = make_float4(e1.y, e1.z, e2.x, e2.y);
+    r[2] = make_float4(e2.z, v1.x, v1.y, v1.z);
+    r[3] = make_float4(v2.x, v2.y, v2.z, 0.f);
+}
+
+SG_D void load_tri_record(const float4* __restrict__ rec, uint32_t i, v3& v0, v3& v1, v3& v2) {
+    const float4* r = rec + 4 * (size_t)i;
+    const float4 a = r[0], c = r[2], d = r[3];
+    v0 = mk(a.x, a.y, a.z);
+    v1 = mk(c.y, c.z, c.w);
+    v2 = mk(d.x, d.y, d.z);
+}
 
 // seven SoA arrays: ox, oy, oz, dx, dy, dz, tmax (device pointers)
 struct RaySoA {
@@ -31,22 +54,38 @@ struct GridView {
     const uint32_t* cells;     // uniform: 2*C ranges; two-level: 2*C1 packed cells
     const uint32_t* leaves;    // two-level: 2*C2 ranges
     const uint32_t* indices;   // reference -> triangle id
-    const float4* rec;         // per-triangle traversal record (3 x float4)
+    const float4* rec;         // per-triangle record (4 x float4, see TriSoA)
     int kind;                  // 0 uniform, 1 two-level
 };
 
 // ---------------------------------------------------------------- build
 void launch_init(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, hipStream_t st);
-void launch_bounds(const TriSoA& t, uint32_t n, GridHeader* h, hipStream_t st);
+// per-block bounds partials (6 words per block, bounds_blocks(n) blocks) and
+// the triangle records
+size_t bounds_blocks(uint32_t n);
+void launch_bounds(const TriSoA& t, uint32_t n, uint32_t* part, hipStream_t st);
+// fold the partials into the header; with derive, resolution + cell sizes
+void launch_params(GridHeader* h, const BuildArgs& a, const uint32_t* part, uint32_t n, bool derive,
+                   hipStream_t st);
+// per-tile sums needed by the emission kernels: emit_tiles(n) for n
+// triangles or top pairs, cell_tiles(C1) for the leaf-cell kernels
+size_t emit_tiles(uint32_t n);
+size_t cell_tiles(uint32_t n);
 void launch_top_emit(const TriSoA& t, uint32_t n, GridHeader* h, const BuildArgs& a, uint32_t* keys,
-                     uint32_t* vals, uint32_t* dbg, uint64_t* lb, hipStream_t st);
+                     uint32_t* vals, uint32_t* dbg, uint32_t* sums, hipStream_t st);
 void launch_ranges(const uint32_t* keys, const uint32_t* d_n, uint32_t cap, const uint32_t* d_C,
                    uint32_t* cells, uint32_t* d_pvalid, hipStream_t st);
 void launch_leaf_cells(GridHeader* h, const BuildArgs& a, const uint32_t* ranges, uint32_t* cells,
-                       uint32_t* dbg, uint64_t* lb, hipStream_t st);
+                       uint32_t* dbg, uint32_t* sums, hipStream_t st);
+// leaf pairs, sorted by key within each leaf tile; sums: 2 * leaf_tiles(cap_P1) + 2 words
+size_t leaf_tiles(uint32_t n);
 void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const uint32_t* top_keys,
-                      const uint32_t* top_vals, const uint32_t* cells, uint32_t* keys, uint32_t* vals,
-                      uint32_t* dbg, uint64_t* lb, hipStream_t st);
+                      const uint32_t* top_vals, const uint32_t* ranges, const uint32_t* cells, uint32_t* keys,
+                      uint32_t* vals, uint32_t* dbg, uint32_t* sums, hipStream_t st);
+// global stable sort of the leaf pairs of tiles too large for LDS (no work
+// when there are none); the result stays in (k0, v0)
+void launch_leaf_sort_fixup(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, GridHeader* h, uint32_t cap,
+                            int key_bits, int radix_bits, uint32_t* scratch, hipStream_t st);
 
 // ---------------------------------------------------------------- sort / scan
 // Scratch words the sort needs for `cap` pairs sorted over `key_bits` bits.
@@ -54,21 +93,18 @@ size_t sort_scratch_words(uint32_t cap, int key_bits, int radix_bits);
 int sort_passes(int key_bits, int radix_bits);
 // Stable LSD radix sort of (keys,vals) by the low key_bits bits.  n is read
 // from device memory (*d_n, clamped to cap).  Input in (k0,v0); the result
-// is in buffer index (sort_passes() & 1) of {(k0,v0),(k1,v1)}.  scratch must
-// be zero (k_init zeroes it) and holds sort_scratch_words() words;
-// tile_ctr points at sort_passes() free, zeroed counters.
+// is in buffer index (sort_passes() & 1) of {(k0,v0),(k1,v1)}.  scratch holds
+// sort_scratch_words() words (no initialisation needed).
 void launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, const uint32_t* d_n,
-                       uint32_t cap, int key_bits, int radix_bits, uint32_t* scratch, uint32_t* tile_ctr,
-                       uint32_t* status, hipStream_t st);
+                       uint32_t cap, int key_bits, int radix_bits, uint32_t* scratch, hipStream_t st);
 
-// single-pass scan of n u32 values (decoupled look-back); scratch: scan_scratch_bytes(n),
-// zeroed by the call itself.
+// scan of n u32 values (reduce-then-scan); scratch: scan_scratch_bytes(n)
 size_t scan_scratch_bytes(uint32_t n);
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, void* scratch,
                  hipStream_t st);
 
 // ---------------------------------------------------------------- traversal
-void launch_prep_tris(const TriSoA& t, uint32_t n, float4* rec, hipStream_t st);
+void launch_prep_tris(const TriSoA& t, uint32_t n, hipStream_t st);
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim,
                   float* t, uint32_t* stats, hipStream_t st);
 // one cosine-weighted bounce per hit, compacted in primary order; *d_count

@@ -4,31 +4,29 @@
 // One ray per lane.  Rays are SoA (coalesced loads), grid cells / leaves /
 // indices are plain buffers (the reference's texture fetches replaced by
 // explicit clamp-to-edge and zero-out-of-range reads), and each triangle is
-// stored as a 48-byte traversal record {v0, e1 = v1-v0, e2 = v2-v0}: the
-// edge subtractions are the same binary32 operations the intersector would
-// perform, so results are unchanged and every Moller-Trumbore test needs two
-// 16-byte loads plus one 4-byte load after the index fetch.
+// read from its 64-byte record, whose first 36 bytes are {v0, e1 = v1-v0,
+// e2 = v2-v0}: the edge subtractions are the same binary32 operations the
+// intersector would perform, so results are unchanged and every
+// Moller-Trumbore test needs two 16-byte loads plus one 4-byte load from one
+// aligned record after the index fetch.
+#include <stdlib.h>
+
 #include "sg_device.hpp"
 #include "sg_kernels.hpp"
 
 namespace sg {
 
 // ---------------------------------------------------------- triangle records
-__global__ __launch_bounds__(kBlock) void k_prep_tris(TriSoA t, uint32_t n, float4* __restrict__ rec) {
+__global__ __launch_bounds__(kBlock) void k_prep_tris(TriSoA t, uint32_t n) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const v3 v0 = mk(t.v[0][i], t.v[1][i], t.v[2][i]);
-    const v3 v1 = mk(t.v[3][i], t.v[4][i], t.v[5][i]);
-    const v3 v2 = mk(t.v[6][i], t.v[7][i], t.v[8][i]);
-    const v3 e1 = v1 - v0, e2 = v2 - v0;
-    rec[3 * (size_t)i + 0] = make_float4(v0.x, v0.y, v0.z, e1.x);
-    rec[3 * (size_t)i + 1] = make_float4(e1.y, e1.z, e2.x, e2.y);
-    rec[3 * (size_t)i + 2] = make_float4(e2.z, 0.f, 0.f, 0.f);
+    store_tri_record(t.rec, i, mk(t.v[0][i], t.v[1][i], t.v[2][i]), mk(t.v[3][i], t.v[4][i], t.v[5][i]),
+                     mk(t.v[6][i], t.v[7][i], t.v[8][i]));
 }
 
-void launch_prep_tris(const TriSoA& t, uint32_t n, float4* rec, hipStream_t st) {
+void launch_prep_tris(const TriSoA& t, uint32_t n, hipStream_t st) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_prep_tris, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, t, n, rec);
+    hipLaunchKernelGGL(k_prep_tris, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, t, n);
 }
 
 // ---------------------------------------------------------------- traversal
@@ -46,9 +44,9 @@ SG_D void intersect(const uint32_t* __restrict__ indices, const float4* __restri
                     uint32_t e, Ray& r) {
     for (uint32_t it = b; it != e; ++it) {
         const uint32_t tri = indices[it];
-        const float4 q0 = rec[3 * (size_t)tri];
-        const float4 q1 = rec[3 * (size_t)tri + 1];
-        const float q2 = reinterpret_cast<const float*>(rec + 3 * (size_t)tri + 2)[0];
+        const float4 q0 = rec[4 * (size_t)tri];
+        const float4 q1 = rec[4 * (size_t)tri + 1];
+        const float q2 = reinterpret_cast<const float*>(rec + 4 * (size_t)tri + 2)[0];
         const v3 v0 = mk(q0.x, q0.y, q0.z);
         const v3 e1 = mk(q0.w, q1.x, q1.y);
         const v3 e2 = mk(q1.z, q1.w, q2);
@@ -247,16 +245,185 @@ __global__ __launch_bounds__(kBlock) void k_trace(GridView g, RaySoA rays, uint3
     }
 }
 
+// ------------------------------------------------------- pooled traversal
+// The same per-ray traversal, scheduled for wave64: each wave owns a chunk of
+// consecutive rays and a lane whose ray has finished takes the next ray of
+// the chunk at the next top-level step (ray index from a wave-uniform
+// counter and a ballot prefix; no global atomics).  Lanes do not sit idle
+// while a long ray in the same wave crosses the grid, which is what limits
+// the one-ray-per-lane kernel on grazing and escaping rays.
+struct TopState {
+    float tEntry;
+    v3 cell, tMax;
+    uint32_t guard, s1, s2, s3;
+};
+
+// one step of the top-level loop (TwoLevelGridTraverser.h:471-581) or of the
+// uniform loop (SimpleGridTraverser.h:294-320); returns false when the ray
+// is done
+template <int KIND>
+SG_D bool top_iteration(const GridView& g, const GParams& p, Ray& r, TopState& s, uint32_t cap) {
+    const uint32_t k = clamp_idx(s.cell.x, p.res[0]) + clamp_idx(s.cell.y, p.res[1]) * (uint32_t)p.res[0] +
+                       clamp_idx(s.cell.z, p.res[2]) * (uint32_t)p.res[0] * (uint32_t)p.res[1];
+    const uint2 top = *reinterpret_cast<const uint2*>(g.cells + 2 * (size_t)k);
+    if (KIND == 0) {
+        intersect(g.indices, g.rec, top.x, top.y, r);
+    } else if (top.y & kNotEmpty) {
+        ++s.s3;
+        const v3 ep = r.o + mk(s.tEntry, s.tEntry, s.tEntry) / r.rcp;
+        const uint32_t c0 = top.y & 0xFFu, c1 = (top.y >> 8) & 0xFFu, c2 = (top.y >> 16) & 0xFFu;
+        const v3 cres = mk((float)c0, (float)c1, (float)c2);
+        const v3 srcp = p.crcp * cres;
+        const v3 mb = s.cell * p.cs + p.bmin;
+        v3 sc = mk(floorf((ep.x - mb.x) * srcp.x), floorf((ep.y - mb.y) * srcp.y), floorf((ep.z - mb.z) * srcp.z));
+        const v3 scs = mk(1.f / srcp.x, 1.f / srcp.y, 1.f / srcp.z);
+        sc = vmin(cres - mk(1.f, 1.f, 1.f), vmax(sc, mk(0.f, 0.f, 0.f)));
+        v3 stMax;
+        stMax.x = ((sc.x + (float)(r.rcp.x > 0.f ? 1 : 0)) * scs.x + mb.x - r.o.x) * r.rcp.x;
+        stMax.y = ((sc.y + (float)(r.rcp.y > 0.f ? 1 : 0)) * scs.y + mb.y - r.o.y) * r.rcp.y;
+        stMax.z = ((sc.z + (float)(r.rcp.z > 0.f ? 1 : 0)) * scs.z + mb.z - r.o.z) * r.rcp.z;
+        bool flag2 = true;
+        const uint32_t icap = 2u * (c0 + c1 + c2) + 16u;
+        uint32_t iguard = 0;
+        while (flag2 && iguard++ < icap) {
+            const float lf = sc.x + (float)c0 * sc.y + (float)c0 * (float)c1 * sc.z;
+            const uint32_t li = top.x + f2u(lf);
+            uint2 rg = make_uint2(0u, 0u);
+            if (li < p.C2) rg = *reinterpret_cast<const uint2*>(g.leaves + 2 * (size_t)li);
+            intersect(g.indices, g.rec, rg.x, rg.y, r);
+            flag2 = r.t > stMax.x || r.t > stMax.y || r.t > stMax.z;
+            if (!flag2) break;
+            ++s.s1;
+            const int d = min_dim(stMax);
+            const float rc = comp(r.rcp, d);
+            setc(sc, d, comp(sc, d) + (rc > 0.f ? 1.f : -1.f));
+            setc(stMax, d, comp(stMax, d) + comp(scs, d) * fabsf(rc));
+            const float sd = comp(sc, d);
+            flag2 = !(fabsf(sd - comp(cres, d)) < 0.1f || fabsf(sd + 1.f) < 0.1f);
+        }
+    }
+    if (!(r.t > s.tMax.x || r.t > s.tMax.y || r.t > s.tMax.z)) return false;
+    if (KIND == 0) ++s.s1;
+    else ++s.s2;
+    const int d = min_dim(s.tMax);
+    const float rc = comp(r.rcp, d);
+    if (KIND == 1) s.tEntry = comp(s.tMax, d);
+    setc(s.cell, d, comp(s.cell, d) + (rc > 0.f ? 1.f : -1.f));
+    setc(s.tMax, d, comp(s.tMax, d) + comp(p.cs, d) * fabsf(rc));
+    const float cd = comp(s.cell, d);
+    if (fabsf(cd - (float)p.res[d]) < 0.1f || fabsf(cd + 1.f) < 0.1f) return false;
+    return ++s.guard < cap;
+}
+
+template <bool STATS>
+SG_D void store_ray(const GridView& g, const Ray& r, const TopState& s, uint32_t id, uint32_t* __restrict__ prim,
+                    float* __restrict__ tout, uint32_t* __restrict__ stats) {
+    prim[id] = r.best == kMiss ? kMiss : g.indices[r.best];
+    tout[id] = r.t;
+    if (STATS) {
+        stats[4 * (size_t)id + 0] = r.tests;
+        stats[4 * (size_t)id + 1] = s.s1;
+        stats[4 * (size_t)id + 2] = s.s2;
+        stats[4 * (size_t)id + 3] = s.s3;
+    }
+}
+
+template <int KIND, bool STATS>
+__global__ __launch_bounds__(kBlock) void k_trace_pool(GridView g, RaySoA rays, uint32_t cap, const uint32_t* d_n,
+                                                       uint32_t* __restrict__ prim, float* __restrict__ tout,
+                                                       uint32_t* __restrict__ stats, uint32_t chunk) {
+    const uint32_t n = d_n ? min(*d_n, cap) : cap;
+    uint32_t next = (blockIdx.x * (kBlock / 64) + (uint32_t)wave_id()) * chunk;
+    if (next >= n) return;
+    const uint32_t end = min(n, next + chunk);
+    const int l = lane_id();
+    if (g.hdr->status & ~(ST_LEAF_RES | ST_NEG_EXTENT)) {
+        for (uint32_t i = next + l; i < end; i += 64) {
+            prim[i] = kMiss;
+            tout[i] = rays.a[6][i];
+            if (STATS)
+                for (int k = 0; k < 4; ++k) stats[4 * (size_t)i + k] = 0;
+        }
+        return;
+    }
+    const GParams p = load_gparams(g.hdr);
+    const uint32_t cap_steps = 2u * (uint32_t)(p.res[0] + p.res[1] + p.res[2]) + 16u;
+    const uint64_t lt = lanemask_lt();
+    bool has = false;
+    uint32_t id = 0;
+    Ray r;
+    TopState s;
+    while (true) {
+        // hand the next rays of the chunk to lanes without one; a ray that
+        // never enters the grid is stored at once and the lane asks again
+        while (true) {
+            const uint64_t need = __ballot(!has);
+            if (need == 0 || next >= end) break;
+            const uint32_t my = next + (uint32_t)__popcll(need & lt);
+            next += (uint32_t)__popcll(need);
+            if (!has && my < end) {
+                id = my;
+                r.o = mk(rays.a[0][id], rays.a[1][id], rays.a[2][id]);
+                const v3 d = mk(rays.a[3][id], rays.a[4][id], rays.a[5][id]);
+                r.rcp = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+                r.D = mk(1.f / r.rcp.x, 1.f / r.rcp.y, 1.f / r.rcp.z);
+                r.t = rays.a[6][id];
+                r.best = kMiss;
+                r.tests = 0;
+                s.guard = s.s1 = s.s2 = s.s3 = 0;
+                bool ok;
+                if (KIND == 0) ok = (r.t > 0.f) && traverse_init<false>(p, r, s.tEntry, s.cell, s.tMax);
+                else ok = traverse_init<true>(p, r, s.tEntry, s.cell, s.tMax) && (r.t > 0.f);
+                if (ok) has = true;
+                else store_ray<STATS>(g, r, s, id, prim, tout, stats);
+            }
+        }
+        if (__ballot(has) == 0) break;
+        if (has && !top_iteration<KIND>(g, p, r, s, cap_steps)) {
+            store_ray<STATS>(g, r, s, id, prim, tout, stats);
+            has = false;
+        }
+    }
+}
+
+static uint32_t trace_chunk() {
+    static uint32_t c = 0;
+    if (c == 0) {
+        const char* e = getenv("SHINOBI_TRACE_CHUNK");
+        c = e ? (uint32_t)atoi(e) : 256u;
+        if (c < 64) c = 64;
+        c = (c + 63) / 64 * 64;
+    }
+    return c;
+}
+
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim, float* t,
                   uint32_t* stats, hipStream_t st) {
     if (n == 0) return;
-    const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
+    // one ray per lane keeps the 64 lanes of a wave on neighbouring rays, so
+    // coherent rays share cell, leaf and triangle loads; the pooled variant
+    // (SHINOBI_TRACE_POOL=1) refills idle lanes but loses that sharing
+    const char* pool = getenv("SHINOBI_TRACE_POOL");
+    if (!(pool && pool[0] == '1')) {
+        const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
+        if (g.kind == 0) {
+            if (stats) hipLaunchKernelGGL((k_trace<0, true>), grid, block, 0, st, g, r, n, d_n, prim, t, stats);
+            else hipLaunchKernelGGL((k_trace<0, false>), grid, block, 0, st, g, r, n, d_n, prim, t, stats);
+        } else {
+            if (stats) hipLaunchKernelGGL((k_trace<1, true>), grid, block, 0, st, g, r, n, d_n, prim, t, stats);
+            else hipLaunchKernelGGL((k_trace<1, false>), grid, block, 0, st, g, r, n, d_n, prim, t, stats);
+        }
+        return;
+    }
+    const uint32_t chunk = trace_chunk();
+    const uint32_t waves = (n + chunk - 1) / chunk;
+    const dim3 grid((waves + kBlock / 64 - 1) / (kBlock / 64)), block(kBlock);
     if (g.kind == 0) {
-        if (stats) hipLaunchKernelGGL((k_trace<0, true>), grid, block, 0, st, g, r, n, d_n, prim, t, stats);
-        else hipLaunchKernelGGL((k_trace<0, false>), grid, block, 0, st, g, r, n, d_n, prim, t, stats);
+        if (stats) hipLaunchKernelGGL((k_trace_pool<0, true>), grid, block, 0, st, g, r, n, d_n, prim, t, stats, chunk);
+        else hipLaunchKernelGGL((k_trace_pool<0, false>), grid, block, 0, st, g, r, n, d_n, prim, t, stats, chunk);
     } else {
-        if (stats) hipLaunchKernelGGL((k_trace<1, true>), grid, block, 0, st, g, r, n, d_n, prim, t, stats);
-        else hipLaunchKernelGGL((k_trace<1, false>), grid, block, 0, st, g, r, n, d_n, prim, t, stats);
+        if (stats) hipLaunchKernelGGL((k_trace_pool<1, true>), grid, block, 0, st, g, r, n, d_n, prim, t, stats, chunk);
+        else hipLaunchKernelGGL((k_trace_pool<1, false>), grid, block, 0, st, g, r, n, d_n, prim, t, stats, chunk);
     }
 }
 
@@ -265,32 +432,60 @@ void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t
 // without Russian roulette): frame from the hit triangle's normalised edge1
 // and n x edge1, n flipped against the ray, origin offset 0.001 n, cosine
 // sample (src/Utils/HemisphereSamplers.hpp:54-72) from a counter hash of the
-// primary ray id.  Compacted in primary order with a decoupled look-back.
+// primary ray id.  Compacted in primary order: per-tile hit counts, a
+// one-block scan of the counts, then each tile writes at its offset.
+constexpr int kBounceTile = kBlock;
+
+__global__ __launch_bounds__(kBlock) void k_bounce_count(const uint32_t* __restrict__ prim, uint32_t n,
+                                                         uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_red[kBlock / 64];
+    const uint32_t i = blockIdx.x * kBounceTile + threadIdx.x;
+    const bool hit = i < n && prim[i] != kMiss;
+    const uint32_t s = block_sum<kBlock / 64>(hit ? 1u : 0u, s_red);
+    if (threadIdx.x == 0) sums[blockIdx.x] = s;
+}
+
+// exclusive scan of the tile counts in place; the total goes to *d_count
+__global__ __launch_bounds__(1024) void k_bounce_scan(uint32_t* sums, uint32_t tiles, uint32_t* d_count) {
+    __shared__ uint32_t s_scan[16];
+    uint32_t run = 0;
+    for (uint32_t c0 = 0; c0 < tiles; c0 += 1024 * 4) {
+        const uint32_t t0 = c0 + threadIdx.x * 4;
+        uint32_t v[4], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[k] = t0 + k < tiles ? sums[t0 + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t tot;
+        uint32_t pre = run + block_excl_scan<16>(sum, &tot, s_scan);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (t0 + k < tiles) sums[t0 + k] = pre;
+            pre += v[k];
+        }
+        run += tot;
+    }
+    if (threadIdx.x == 0) *d_count = run;
+}
+
 __global__ __launch_bounds__(kBlock) void k_bounce(TriSoA tris, RaySoA rays, const uint32_t* __restrict__ prim,
                                                    const float* __restrict__ tin, uint32_t n, uint64_t seed,
-                                                   RaySoAOut out, uint32_t* __restrict__ src, uint32_t* d_count,
-                                                   uint64_t* scratch) {
+                                                   RaySoAOut out, uint32_t* __restrict__ src,
+                                                   const uint32_t* __restrict__ sums) {
     __shared__ uint32_t s_scan[kBlock / 64];
-    __shared__ uint32_t s_word;
-    uint32_t* ctr = reinterpret_cast<uint32_t*>(scratch);
-    uint32_t* status = ctr + 1;
-    uint64_t* lb = scratch + 2;
-    const uint32_t tile = next_tile(ctr, &s_word);
-    const uint32_t i = tile * kBlock + threadIdx.x;
+    const uint32_t i = blockIdx.x * kBounceTile + threadIdx.x;
     const bool hit = i < n && prim[i] != kMiss;
     uint32_t agg;
     const uint32_t local = block_excl_scan<kBlock / 64>(hit ? 1u : 0u, &agg, s_scan);
-    const uint32_t prefix = lb_tile_prefix(lb, tile, agg, status, &s_word);
-    if (tile == (n + kBlock - 1) / kBlock - 1 && threadIdx.x == 0) *d_count = prefix + agg;
     if (!hit) return;
-    const uint32_t m = prefix + local;
+    const uint32_t m = sums[blockIdx.x] + local;
     const uint32_t tri = prim[i];
     const v3 o = mk(rays.a[0][i], rays.a[1][i], rays.a[2][i]);
     const v3 d = mk(rays.a[3][i], rays.a[4][i], rays.a[5][i]);
-    const v3 v0 = mk(tris.v[0][tri], tris.v[1][tri], tris.v[2][tri]);
-    const v3 v1 = mk(tris.v[3][tri], tris.v[4][tri], tris.v[5][tri]);
-    const v3 v2 = mk(tris.v[6][tri], tris.v[7][tri], tris.v[8][tri]);
-    v3 ta = v1 - v0, tb = v2 - v0;
+    const float4* rq = tris.rec + 4 * (size_t)tri;
+    const float4 q0 = rq[0], q1 = rq[1], q2 = rq[2];
+    v3 ta = mk(q0.w, q1.x, q1.y), tb = mk(q1.z, q1.w, q2.x);   // v1 - v0, v2 - v0
     v3 nn = normalize(cross(ta, tb));
     ta = normalize(ta);
     tb = cross(nn, ta);
@@ -312,26 +507,27 @@ __global__ __launch_bounds__(kBlock) void k_bounce(TriSoA tris, RaySoA rays, con
     if (src) src[m] = i;
 }
 
-__global__ void k_bounce_init(uint64_t* p, size_t n, uint32_t* d_count) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *d_count = 0u;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        p[i] = 0;
+__global__ void k_set_u32(uint32_t* p, uint32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 size_t bounce_scratch_bytes(uint32_t n) {
-    const size_t tiles = ((size_t)n + kBlock - 1) / kBlock;
-    return ((tiles + 2) * sizeof(uint64_t) + 15) & ~(size_t)15;
+    const size_t tiles = ((size_t)n + kBounceTile - 1) / kBounceTile;
+    return ((tiles + 4) * sizeof(uint32_t) + 15) & ~(size_t)15;
 }
 
 void launch_bounce(const TriSoA& tris, const RaySoA& r, const uint32_t* prim, const float* t, uint32_t n,
                    uint64_t seed, const RaySoAOut& out, uint32_t* src, uint32_t* d_count, void* scratch,
                    hipStream_t st) {
-    const uint32_t tiles = (n + kBlock - 1) / kBlock;
-    uint64_t* s = reinterpret_cast<uint64_t*>(scratch);
-    hipLaunchKernelGGL(k_bounce_init, dim3((tiles + 2 + 255) / 256), dim3(256), 0, st, s, (size_t)tiles + 2,
-                       d_count);
-    if (n == 0) return;
-    hipLaunchKernelGGL(k_bounce, dim3(tiles), dim3(kBlock), 0, st, tris, r, prim, t, n, seed, out, src, d_count, s);
+    if (n == 0) {
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, d_count, 0u);
+        return;
+    }
+    const uint32_t tiles = (n + kBounceTile - 1) / kBounceTile;
+    uint32_t* sums = reinterpret_cast<uint32_t*>(scratch);
+    hipLaunchKernelGGL(k_bounce_count, dim3(tiles), dim3(kBlock), 0, st, prim, n, sums);
+    hipLaunchKernelGGL(k_bounce_scan, dim3(1), dim3(1024), 0, st, sums, tiles, d_count);
+    hipLaunchKernelGGL(k_bounce, dim3(tiles), dim3(kBlock), 0, st, tris, r, prim, t, n, seed, out, src, sums);
 }
 
 }  // namespace sg
