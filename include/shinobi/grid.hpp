@@ -292,4 +292,12 @@ inline void trace(const GridBase& grid, Context& ctx, const sg_rays& rays, uint3
           "sg_trace_closest");
 }
 
+// The shadow-ray contract (TwoLevelGridShadowTraverser /
+// UniformGridShadowTraverser): segments in, occluded where t < 0.9999.
+inline void traceShadow(const GridBase& grid, Context& ctx, const sg_rays& rays, uint32_t n, uint32_t* d_prim,
+                        float* d_t, void* stream = nullptr) {
+    check(sg_trace_shadow(ctx.get(), grid.handle(), &rays, n, nullptr, d_prim, d_t, nullptr, stream),
+          "sg_trace_shadow");
+}
+
 }  // namespace shinobi

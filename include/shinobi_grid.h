@@ -166,6 +166,17 @@ sg_status sg_grid_destroy(sg_grid *grid);
 sg_status sg_trace_closest(sg_ctx *ctx, sg_grid *grid, const sg_rays *rays, uint32_t n,
                            const uint32_t *d_count, uint32_t *d_prim, float *d_t, uint32_t *d_stats,
                            void *stream);
+/* Any-hit (shadow) traversal: TwoLevelGridShadowTraverser
+ * (src/Control/Algorithm/TwoLevelGridTraverser.h:677-828) or
+ * UniformGridShadowTraverser (src/Control/Algorithm/SimpleGridTraverser.h:409-464)
+ * as bound by DirectIlluminationIntegrator (src/Integrator/DirectIlluminationIntegrator.h:188-189).
+ * Rays are segments (direction = light - origin, tmax = 1); a ray is
+ * occluded when the returned t < 0.9999 (OcclusionRayBuffer::store,
+ * src/Engine/RayBuffers.h:156-167).  prim is the occluder found first
+ * (0xFFFFFFFF when unoccluded).  Same arguments as sg_trace_closest. */
+sg_status sg_trace_shadow(sg_ctx *ctx, sg_grid *grid, const sg_rays *rays, uint32_t n,
+                          const uint32_t *d_count, uint32_t *d_prim, float *d_t, uint32_t *d_stats,
+                          void *stream);
 /* one cosine-weighted diffuse bounce per hit (SimplePathTracer.hpp:115-166),
  * compacted in primary order; *d_count = number of bounce rays; d_src (opt.)
  * = primary ray index of each bounce ray */

@@ -89,6 +89,7 @@ def load() -> C.CDLL:
     lib.or_build_twolevel.argtypes = [P9, C.c_uint32, C.POINTER(Params)]
     lib.or_grid_free.argtypes = [C.POINTER(GridC)]
     lib.or_trace.argtypes = [C.POINTER(GridC), P9, P7, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    lib.or_trace_shadow.argtypes = lib.or_trace.argtypes
     lib.or_brute_f32.argtypes = [P9, C.c_uint32, P7, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int]
     lib.or_brute_f64.argtypes = [P9, C.c_uint32, P7, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int]
     lib.or_bounce.restype = C.c_uint32
@@ -147,14 +148,16 @@ class OracleGrid:
         self.leaves = _u32(g.leaves, 2 * g.C2) if g.kind == 1 else None
         self.indices = _u32(g.indices, g.n_indices)
 
-    def trace(self, rays7: np.ndarray, nthreads: int = 8, with_stats: bool = False):
+    def trace(self, rays7: np.ndarray, nthreads: int = 8, with_stats: bool = False, shadow: bool = False):
+        """Closest hit (or, with shadow=True, the any-hit shadow traversal)."""
         rays7 = np.ascontiguousarray(rays7, np.float32)
         n = rays7.shape[1]
         prim = np.empty(n, np.uint32)
         t = np.empty(n, np.float32)
         stats = np.empty((n, 4), np.uint32) if with_stats else None
-        load().or_trace(self._p, _p9(self._soa9), _p7(rays7), n, prim.ctypes.data, t.ctypes.data,
-                        stats.ctypes.data if with_stats else None, nthreads)
+        fn = load().or_trace_shadow if shadow else load().or_trace
+        fn(self._p, _p9(self._soa9), _p7(rays7), n, prim.ctypes.data, t.ctypes.data,
+           stats.ctypes.data if with_stats else None, nthreads)
         return (prim, t, stats) if with_stats else (prim, t)
 
     def close(self):

@@ -757,9 +757,16 @@ typedef struct {
     const float *const *v;
     const float *const *rays;
     uint32_t *prim; float *t; uint32_t *stats;
+    int shadow;             /* any-hit: stop once t < OR_SHADOW_T */
 } trace_ctx;
 
-/* UniformGridTraverser, src/Control/Algorithm/SimpleGridTraverser.h:274-401 */
+/* occlusion threshold of the shadow traversers, TwoLevelGridTraverser.h:718,
+ * SimpleGridTraverser.h:436 */
+#define OR_SHADOW_T 0.9999f
+
+/* UniformGridTraverser, src/Control/Algorithm/SimpleGridTraverser.h:274-401;
+ * shadow: UniformGridShadowTraverser :409-464 (starts only if t >= 0.9999,
+ * walks to the grid exit and stops once t < 0.9999; no t > tMax test) */
 static void trace_uniform_one(const trace_ctx *c, uint32_t i)
 {
     const or_grid *g = c->g;
@@ -775,7 +782,7 @@ static void trace_uniform_one(const trace_ctx *c, uint32_t i)
     v3 cs = mk(g->cs[0], g->cs[1], g->cs[2]), crcp = mk(g->cell_rcp[0], g->cell_rcp[1], g->cell_rcp[2]);
     int flag = 0;
     v3 cell = mk(0, 0, 0), tMax = mk(0, 0, 0);
-    if (r.t > 0.f) {
+    if (c->shadow ? r.t >= OR_SHADOW_T : r.t > 0.f) {
         /* traverseInit :325-369 with BBox::fastClip, BBox.hpp:60-75 */
         v3 t1 = mul(sub(bmin, r.o), r.rcp), t2 = mul(sub(bmax, r.o), r.rcp);
         v3 tn = vmin(t1, t2), tx = vmax(t1, t2);
@@ -801,8 +808,12 @@ static void trace_uniform_one(const trace_ctx *c, uint32_t i)
                      clamp_idx(cell.y, g->res[1]) * (uint32_t)g->res[0] +
                      clamp_idx(cell.z, g->res[2]) * (uint32_t)g->res[0] * (uint32_t)g->res[1];
         intersect(c->v, g->indices, g->top_ranges[2 * (size_t)k], g->top_ranges[2 * (size_t)k + 1], &r);
-        flag = r.t > tMax.x || r.t > tMax.y || r.t > tMax.z;
-        if (!flag) break;
+        if (c->shadow) {
+            if (!(r.t >= OR_SHADOW_T)) break;
+        } else {
+            flag = r.t > tMax.x || r.t > tMax.y || r.t > tMax.z;
+            if (!flag) break;
+        }
         /* traverseCell :371-401 */
         ++steps;
         int dd = min_dim(tMax);
@@ -820,7 +831,12 @@ static void trace_uniform_one(const trace_ctx *c, uint32_t i)
     }
 }
 
-/* TwoLevelGridTraverser, src/Control/Algorithm/TwoLevelGridTraverser.h:445-671 */
+/* TwoLevelGridTraverser, src/Control/Algorithm/TwoLevelGridTraverser.h:445-671;
+ * shadow: TwoLevelGridShadowTraverser :677-828 (a ray with t < 0.9999 does
+ * not start; every leaf cell of an entered top cell and every top cell up to
+ * the grid exit is visited until t < 0.9999; no t > tMax tests).  The
+ * reference ignores a failed traverseInit there and walks from an
+ * uninitialised cell; here such a ray is not traversed. */
 static void trace_twolevel_one(const trace_ctx *c, uint32_t i)
 {
     const or_grid *g = c->g;
@@ -834,7 +850,7 @@ static void trace_twolevel_one(const trace_ctx *c, uint32_t i)
     uint32_t lsteps = 0, tsteps = 0, entered = 0;
     v3 bmin = mk(g->bmin[0], g->bmin[1], g->bmin[2]), bmax = mk(g->bmax[0], g->bmax[1], g->bmax[2]);
     v3 cs = mk(g->cs[0], g->cs[1], g->cs[2]), crcp = mk(g->cell_rcp[0], g->cell_rcp[1], g->cell_rcp[2]);
-    int flag1 = r.t > 0.f;
+    int flag1 = c->shadow ? !(r.t < OR_SHADOW_T) : r.t > 0.f;
     v3 cell = mk(0, 0, 0), tMax = mk(0, 0, 0);
     float tEntry, tExit;
     {
@@ -894,8 +910,12 @@ static void trace_twolevel_one(const trace_ctx *c, uint32_t i)
                 uint32_t lb = 0, le = 0;
                 if (li < g->C2) { lb = g->leaves[2 * (size_t)li]; le = g->leaves[2 * (size_t)li + 1]; }
                 intersect(c->v, g->indices, lb, le, &r);
-                flag2 = r.t > stMax.x || r.t > stMax.y || r.t > stMax.z;
-                if (!flag2) break;
+                if (c->shadow) {
+                    if (r.t < OR_SHADOW_T) break;
+                } else {
+                    flag2 = r.t > stMax.x || r.t > stMax.y || r.t > stMax.z;
+                    if (!flag2) break;
+                }
                 /* traverseCell :651-671 */
                 ++lsteps;
                 int dd = min_dim(stMax);
@@ -906,8 +926,12 @@ static void trace_twolevel_one(const trace_ctx *c, uint32_t i)
                 flag2 = !(fabsf(sd - comp(cres, dd)) < 0.1f || fabsf(sd + 1.f) < 0.1f);
             }
         }
-        flag1 = r.t > tMax.x || r.t > tMax.y || r.t > tMax.z;
-        if (!flag1) break;
+        if (c->shadow) {
+            if (r.t < OR_SHADOW_T) break;
+        } else {
+            flag1 = r.t > tMax.x || r.t > tMax.y || r.t > tMax.z;
+            if (!flag1) break;
+        }
         /* traverseMacroCell :628-649 */
         ++tsteps;
         int dd = min_dim(tMax);
@@ -939,7 +963,14 @@ static void trace_fn(void *c_, size_t b, size_t e, int tid)
 void or_trace(const or_grid *g, const float *const v[9], const float *const rays[7], uint32_t nrays,
               uint32_t *prim, float *t, uint32_t *stats, int nthreads)
 {
-    trace_ctx c = {g, v, rays, prim, t, stats};
+    trace_ctx c = {g, v, rays, prim, t, stats, 0};
+    parallel_for(nrays, nthreads, trace_fn, &c);
+}
+
+void or_trace_shadow(const or_grid *g, const float *const v[9], const float *const rays[7], uint32_t nrays,
+                     uint32_t *prim, float *t, uint32_t *stats, int nthreads)
+{
+    trace_ctx c = {g, v, rays, prim, t, stats, 1};
     parallel_for(nrays, nthreads, trace_fn, &c);
 }
 

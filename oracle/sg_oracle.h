@@ -110,6 +110,12 @@ void     or_grid_free(or_grid *g);
 void or_trace(const or_grid *g, const float *const v[9],
               const float *const rays[7], uint32_t nrays,
               uint32_t *prim, float *t, uint32_t *stats, int nthreads);
+/* any-hit (shadow) traversal of the same grid: the reference's shadow
+ * traversers, which stop a ray once its t drops below 0.9999 (occluded) and
+ * otherwise walk it to the grid exit; same outputs as or_trace. */
+void or_trace_shadow(const or_grid *g, const float *const v[9],
+                     const float *const rays[7], uint32_t nrays,
+                     uint32_t *prim, float *t, uint32_t *stats, int nthreads);
 
 /* brute force over every triangle, same MT formula, reciprocal-direction form */
 void or_brute_f32(const float *const v[9], uint32_t n,

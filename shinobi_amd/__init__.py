@@ -313,21 +313,24 @@ class Grid:
               "sg_rebuild_phased")
 
     def trace(self, rays: RayBatch, prim: DeviceBuffer | None = None, t: DeviceBuffer | None = None,
-              stats: DeviceBuffer | None = None, d_count: C.c_void_p | None = None, n: int | None = None):
+              stats: DeviceBuffer | None = None, d_count: C.c_void_p | None = None, n: int | None = None,
+              shadow: bool = False):
+        """Closest hit (sg_trace_closest) or, with shadow=True, the any-hit
+        shadow traversal (sg_trace_shadow: occluded where t < 0.9999)."""
         n = rays.n if n is None else n
         prim = prim or DeviceBuffer(self.ctx, 4 * max(n, 1))
         t = t or DeviceBuffer(self.ctx, 4 * max(n, 1))
         r = rays.struct()
-        check(self.ctx._l.sg_trace_closest(self.ctx.h, self.h, C.byref(r), n, d_count, prim.ptr, t.ptr,
-                                           stats.ptr if stats is not None else None, self.ctx.stream),
-              "sg_trace_closest")
+        name = "sg_trace_shadow" if shadow else "sg_trace_closest"
+        check(getattr(self.ctx._l, name)(self.ctx.h, self.h, C.byref(r), n, d_count, prim.ptr, t.ptr,
+                                         stats.ptr if stats is not None else None, self.ctx.stream), name)
         return prim, t
 
-    def trace_host(self, rays7: np.ndarray, with_stats: bool = False):
+    def trace_host(self, rays7: np.ndarray, with_stats: bool = False, shadow: bool = False):
         """Convenience: upload rays, trace, download (prim, t[, stats])."""
         rb = RayBatch(self.ctx, rays7)
         stats = DeviceBuffer(self.ctx, 16 * max(rb.n, 1)) if with_stats else None
-        prim, t = self.trace(rb, stats=stats)
+        prim, t = self.trace(rb, stats=stats, shadow=shadow)
         out = (prim.download(np.uint32, rb.n), t.download(np.float32, rb.n))
         if with_stats:
             out = out + (stats.download(np.uint32, 4 * rb.n).reshape(rb.n, 4),)

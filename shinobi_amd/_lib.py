@@ -90,6 +90,8 @@ SIGNATURES = {
     "sg_grid_destroy": (C.c_int, [C.c_void_p]),
     "sg_trace_closest": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_uint32, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sg_trace_shadow": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_uint32, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_void_p]),
     "sg_bounce": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_uint32,
                             C.c_uint64, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_void_p]),
     "sg_scan_u32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]),

@@ -608,16 +608,26 @@ static RaySoA ray_soa(const sg_rays* r) {
     return s;
 }
 
-sg_status sg_trace_closest(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint32_t n, const uint32_t* d_count,
-                           uint32_t* d_prim, float* d_t, uint32_t* d_stats, void* stream) {
+static sg_status trace_common(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint32_t n, const uint32_t* d_count,
+                              uint32_t* d_prim, float* d_t, uint32_t* d_stats, bool shadow, void* stream) {
     if (!ctx || !g || !rays || !d_prim || !d_t) return SG_ERR_INVALID_ARG;
     for (int i = 0; i < 3; ++i)
         if (!rays->o[i] || !rays->d[i]) return SG_ERR_INVALID_ARG;
     if (!rays->tmax) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
-    launch_trace(grid_view(g), ray_soa(rays), n, d_count, d_prim, d_t, d_stats, (hipStream_t)stream);
+    launch_trace(grid_view(g), ray_soa(rays), n, d_count, d_prim, d_t, d_stats, shadow, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return SG_OK;
+}
+
+sg_status sg_trace_closest(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint32_t n, const uint32_t* d_count,
+                           uint32_t* d_prim, float* d_t, uint32_t* d_stats, void* stream) {
+    return trace_common(ctx, g, rays, n, d_count, d_prim, d_t, d_stats, false, stream);
+}
+
+sg_status sg_trace_shadow(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint32_t n, const uint32_t* d_count,
+                          uint32_t* d_prim, float* d_t, uint32_t* d_stats, void* stream) {
+    return trace_common(ctx, g, rays, n, d_count, d_prim, d_t, d_stats, true, stream);
 }
 
 sg_status sg_bounce(sg_ctx* ctx, sg_scene* scene, const sg_rays* rays, const uint32_t* d_prim, const float* d_t,

@@ -105,8 +105,9 @@ void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, 
 
 // ---------------------------------------------------------------- traversal
 void launch_prep_tris(const TriSoA& t, uint32_t n, hipStream_t st);
+// closest hit, or with shadow the any-hit traversal that stops at t < 0.9999
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim,
-                  float* t, uint32_t* stats, hipStream_t st);
+                  float* t, uint32_t* stats, bool shadow, hipStream_t st);
 // one cosine-weighted bounce per hit, compacted in primary order; *d_count
 // receives the number of bounce rays.  scratch: bounce_scratch_bytes(n).
 size_t bounce_scratch_bytes(uint32_t n);

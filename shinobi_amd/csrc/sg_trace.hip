@@ -9,8 +9,6 @@
 // intersector would perform, so results are unchanged and every
 // Moller-Trumbore test needs two 16-byte loads plus one 4-byte load from one
 // aligned record after the index fetch.
-#include <stdlib.h>
-
 #include "sg_device.hpp"
 #include "sg_kernels.hpp"
 
@@ -119,11 +117,18 @@ SG_D bool traverse_init(const GParams& p, const Ray& r, float& tEntry, v3& cell,
            (cell.z != (float)(r.rcp.z > 0.f ? p.res[2] : -1));
 }
 
-// UniformGridTraverser::traverse, src/Control/Algorithm/SimpleGridTraverser.h:274-401
+// occlusion threshold of the shadow traversers (TwoLevelGridTraverser.h:718,
+// SimpleGridTraverser.h:436)
+constexpr float kShadowT = 0.9999f;
+
+// UniformGridTraverser::traverse, src/Control/Algorithm/SimpleGridTraverser.h:274-401;
+// SHADOW: UniformGridShadowTraverser (:409-464) starts only if t >= 0.9999,
+// stops once t < 0.9999 and otherwise walks to the grid exit
+template <bool SHADOW>
 SG_D void trace_uniform(const GridView& g, const GParams& p, Ray& r, uint32_t& steps) {
     float tEntry;
     v3 cell, tMax;
-    bool flag = (r.t > 0.f) && traverse_init<false>(p, r, tEntry, cell, tMax);
+    bool flag = (SHADOW ? r.t >= kShadowT : r.t > 0.f) && traverse_init<false>(p, r, tEntry, cell, tMax);
     // a ray crosses at most res.x+res.y+res.z cells; the cap only ends walks
     // that can never reach the exit test (NaN / inf rays, entry cell past
     // the far boundary), which the reference would loop on forever
@@ -134,8 +139,12 @@ SG_D void trace_uniform(const GridView& g, const GParams& p, Ray& r, uint32_t& s
                            clamp_idx(cell.z, p.res[2]) * (uint32_t)p.res[0] * (uint32_t)p.res[1];
         const uint2 rg = *reinterpret_cast<const uint2*>(g.cells + 2 * (size_t)k);
         intersect(g.indices, g.rec, rg.x, rg.y, r);
-        flag = r.t > tMax.x || r.t > tMax.y || r.t > tMax.z;
-        if (!flag) break;
+        if (SHADOW) {
+            if (!(r.t >= kShadowT)) break;
+        } else {
+            flag = r.t > tMax.x || r.t > tMax.y || r.t > tMax.z;
+            if (!flag) break;
+        }
         ++steps;
         const int d = min_dim(tMax);
         const float rc = comp(r.rcp, d);
@@ -146,12 +155,18 @@ SG_D void trace_uniform(const GridView& g, const GParams& p, Ray& r, uint32_t& s
     }
 }
 
-// TwoLevelGridTraverser::traverse, src/Control/Algorithm/TwoLevelGridTraverser.h:445-671
+// TwoLevelGridTraverser::traverse, src/Control/Algorithm/TwoLevelGridTraverser.h:445-671;
+// SHADOW: TwoLevelGridShadowTraverser (:677-828) does not start a ray with
+// t < 0.9999, visits every leaf cell of an entered top cell and every top
+// cell up to the grid exit until t < 0.9999 (no t > tMax tests).  It ignores
+// a failed traverseInit and walks from an uninitialised cell; here such a
+// ray is not traversed.
+template <bool SHADOW>
 SG_D void trace_twolevel(const GridView& g, const GParams& p, Ray& r, uint32_t& lsteps, uint32_t& tsteps,
                          uint32_t& entered) {
     float tEntry;
     v3 cell, tMax;
-    bool flag1 = r.t > 0.f;
+    bool flag1 = SHADOW ? !(r.t < kShadowT) : r.t > 0.f;
     if (!traverse_init<true>(p, r, tEntry, cell, tMax)) flag1 = false;
     const uint32_t cap = 2u * (uint32_t)(p.res[0] + p.res[1] + p.res[2]) + 16u;
     uint32_t guard = 0;
@@ -182,8 +197,12 @@ SG_D void trace_twolevel(const GridView& g, const GParams& p, Ray& r, uint32_t& 
                 uint2 rg = make_uint2(0u, 0u);
                 if (li < p.C2) rg = *reinterpret_cast<const uint2*>(g.leaves + 2 * (size_t)li);
                 intersect(g.indices, g.rec, rg.x, rg.y, r);
-                flag2 = r.t > stMax.x || r.t > stMax.y || r.t > stMax.z;
-                if (!flag2) break;
+                if (SHADOW) {
+                    if (r.t < kShadowT) break;
+                } else {
+                    flag2 = r.t > stMax.x || r.t > stMax.y || r.t > stMax.z;
+                    if (!flag2) break;
+                }
                 ++lsteps;
                 const int d = min_dim(stMax);
                 const float rc = comp(r.rcp, d);
@@ -193,8 +212,12 @@ SG_D void trace_twolevel(const GridView& g, const GParams& p, Ray& r, uint32_t& 
                 flag2 = !(fabsf(sd - comp(cres, d)) < 0.1f || fabsf(sd + 1.f) < 0.1f);
             }
         }
-        flag1 = r.t > tMax.x || r.t > tMax.y || r.t > tMax.z;
-        if (!flag1) break;
+        if (SHADOW) {
+            if (r.t < kShadowT) break;
+        } else {
+            flag1 = r.t > tMax.x || r.t > tMax.y || r.t > tMax.z;
+            if (!flag1) break;
+        }
         ++tsteps;
         const int d = min_dim(tMax);
         const float rc = comp(r.rcp, d);
@@ -208,7 +231,7 @@ SG_D void trace_twolevel(const GridView& g, const GParams& p, Ray& r, uint32_t& 
 
 // the trace kernel of src/Engine/RayTracingKernels.h:37-136 for SoA rays:
 // rcp(dir) (:111-113), traverse, store prim = indices[best] (miss: ~0u) and t
-template <int KIND, bool STATS>
+template <int KIND, bool STATS, bool SHADOW>
 __global__ __launch_bounds__(kBlock) void k_trace(GridView g, RaySoA rays, uint32_t cap, const uint32_t* d_n,
                                                   uint32_t* __restrict__ prim, float* __restrict__ tout,
                                                   uint32_t* __restrict__ stats) {
@@ -233,8 +256,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(GridView g, RaySoA rays, uint3
     r.best = kMiss;
     r.tests = 0;
     uint32_t s1 = 0, s2 = 0, s3 = 0;
-    if (KIND == 0) trace_uniform(g, p, r, s1);
-    else trace_twolevel(g, p, r, s1, s2, s3);
+    if (KIND == 0) trace_uniform<SHADOW>(g, p, r, s1);
+    else trace_twolevel<SHADOW>(g, p, r, s1, s2, s3);
     prim[i] = r.best == kMiss ? kMiss : g.indices[r.best];
     tout[i] = r.t;
     if (STATS) {
@@ -245,186 +268,21 @@ __global__ __launch_bounds__(kBlock) void k_trace(GridView g, RaySoA rays, uint3
     }
 }
 
-// ------------------------------------------------------- pooled traversal
-// The same per-ray traversal, scheduled for wave64: each wave owns a chunk of
-// consecutive rays and a lane whose ray has finished takes the next ray of
-// the chunk at the next top-level step (ray index from a wave-uniform
-// counter and a ballot prefix; no global atomics).  Lanes do not sit idle
-// while a long ray in the same wave crosses the grid, which is what limits
-// the one-ray-per-lane kernel on grazing and escaping rays.
-struct TopState {
-    float tEntry;
-    v3 cell, tMax;
-    uint32_t guard, s1, s2, s3;
-};
-
-// one step of the top-level loop (TwoLevelGridTraverser.h:471-581) or of the
-// uniform loop (SimpleGridTraverser.h:294-320); returns false when the ray
-// is done
-template <int KIND>
-SG_D bool top_iteration(const GridView& g, const GParams& p, Ray& r, TopState& s, uint32_t cap) {
-    const uint32_t k = clamp_idx(s.cell.x, p.res[0]) + clamp_idx(s.cell.y, p.res[1]) * (uint32_t)p.res[0] +
-                       clamp_idx(s.cell.z, p.res[2]) * (uint32_t)p.res[0] * (uint32_t)p.res[1];
-    const uint2 top = *reinterpret_cast<const uint2*>(g.cells + 2 * (size_t)k);
-    if (KIND == 0) {
-        intersect(g.indices, g.rec, top.x, top.y, r);
-    } else if (top.y & kNotEmpty) {
-        ++s.s3;
-        const v3 ep = r.o + mk(s.tEntry, s.tEntry, s.tEntry) / r.rcp;
-        const uint32_t c0 = top.y & 0xFFu, c1 = (top.y >> 8) & 0xFFu, c2 = (top.y >> 16) & 0xFFu;
-        const v3 cres = mk((float)c0, (float)c1, (float)c2);
-        const v3 srcp = p.crcp * cres;
-        const v3 mb = s.cell * p.cs + p.bmin;
-        v3 sc = mk(floorf((ep.x - mb.x) * srcp.x), floorf((ep.y - mb.y) * srcp.y), floorf((ep.z - mb.z) * srcp.z));
-        const v3 scs = mk(1.f / srcp.x, 1.f / srcp.y, 1.f / srcp.z);
-        sc = vmin(cres - mk(1.f, 1.f, 1.f), vmax(sc, mk(0.f, 0.f, 0.f)));
-        v3 stMax;
-        stMax.x = ((sc.x + (float)(r.rcp.x > 0.f ? 1 : 0)) * scs.x + mb.x - r.o.x) * r.rcp.x;
-        stMax.y = ((sc.y + (float)(r.rcp.y > 0.f ? 1 : 0)) * scs.y + mb.y - r.o.y) * r.rcp.y;
-        stMax.z = ((sc.z + (float)(r.rcp.z > 0.f ? 1 : 0)) * scs.z + mb.z - r.o.z) * r.rcp.z;
-        bool flag2 = true;
-        const uint32_t icap = 2u * (c0 + c1 + c2) + 16u;
-        uint32_t iguard = 0;
-        while (flag2 && iguard++ < icap) {
-            const float lf = sc.x + (float)c0 * sc.y + (float)c0 * (float)c1 * sc.z;
-            const uint32_t li = top.x + f2u(lf);
-            uint2 rg = make_uint2(0u, 0u);
-            if (li < p.C2) rg = *reinterpret_cast<const uint2*>(g.leaves + 2 * (size_t)li);
-            intersect(g.indices, g.rec, rg.x, rg.y, r);
-            flag2 = r.t > stMax.x || r.t > stMax.y || r.t > stMax.z;
-            if (!flag2) break;
-            ++s.s1;
-            const int d = min_dim(stMax);
-            const float rc = comp(r.rcp, d);
-            setc(sc, d, comp(sc, d) + (rc > 0.f ? 1.f : -1.f));
-            setc(stMax, d, comp(stMax, d) + comp(scs, d) * fabsf(rc));
-            const float sd = comp(sc, d);
-            flag2 = !(fabsf(sd - comp(cres, d)) < 0.1f || fabsf(sd + 1.f) < 0.1f);
-        }
-    }
-    if (!(r.t > s.tMax.x || r.t > s.tMax.y || r.t > s.tMax.z)) return false;
-    if (KIND == 0) ++s.s1;
-    else ++s.s2;
-    const int d = min_dim(s.tMax);
-    const float rc = comp(r.rcp, d);
-    if (KIND == 1) s.tEntry = comp(s.tMax, d);
-    setc(s.cell, d, comp(s.cell, d) + (rc > 0.f ? 1.f : -1.f));
-    setc(s.tMax, d, comp(s.tMax, d) + comp(p.cs, d) * fabsf(rc));
-    const float cd = comp(s.cell, d);
-    if (fabsf(cd - (float)p.res[d]) < 0.1f || fabsf(cd + 1.f) < 0.1f) return false;
-    return ++s.guard < cap;
-}
-
-template <bool STATS>
-SG_D void store_ray(const GridView& g, const Ray& r, const TopState& s, uint32_t id, uint32_t* __restrict__ prim,
-                    float* __restrict__ tout, uint32_t* __restrict__ stats) {
-    prim[id] = r.best == kMiss ? kMiss : g.indices[r.best];
-    tout[id] = r.t;
-    if (STATS) {
-        stats[4 * (size_t)id + 0] = r.tests;
-        stats[4 * (size_t)id + 1] = s.s1;
-        stats[4 * (size_t)id + 2] = s.s2;
-        stats[4 * (size_t)id + 3] = s.s3;
-    }
-}
-
-template <int KIND, bool STATS>
-__global__ __launch_bounds__(kBlock) void k_trace_pool(GridView g, RaySoA rays, uint32_t cap, const uint32_t* d_n,
-                                                       uint32_t* __restrict__ prim, float* __restrict__ tout,
-                                                       uint32_t* __restrict__ stats, uint32_t chunk) {
-    const uint32_t n = d_n ? min(*d_n, cap) : cap;
-    uint32_t next = (blockIdx.x * (kBlock / 64) + (uint32_t)wave_id()) * chunk;
-    if (next >= n) return;
-    const uint32_t end = min(n, next + chunk);
-    const int l = lane_id();
-    if (g.hdr->status & ~(ST_LEAF_RES | ST_NEG_EXTENT)) {
-        for (uint32_t i = next + l; i < end; i += 64) {
-            prim[i] = kMiss;
-            tout[i] = rays.a[6][i];
-            if (STATS)
-                for (int k = 0; k < 4; ++k) stats[4 * (size_t)i + k] = 0;
-        }
-        return;
-    }
-    const GParams p = load_gparams(g.hdr);
-    const uint32_t cap_steps = 2u * (uint32_t)(p.res[0] + p.res[1] + p.res[2]) + 16u;
-    const uint64_t lt = lanemask_lt();
-    bool has = false;
-    uint32_t id = 0;
-    Ray r;
-    TopState s;
-    while (true) {
-        // hand the next rays of the chunk to lanes without one; a ray that
-        // never enters the grid is stored at once and the lane asks again
-        while (true) {
-            const uint64_t need = __ballot(!has);
-            if (need == 0 || next >= end) break;
-            const uint32_t my = next + (uint32_t)__popcll(need & lt);
-            next += (uint32_t)__popcll(need);
-            if (!has && my < end) {
-                id = my;
-                r.o = mk(rays.a[0][id], rays.a[1][id], rays.a[2][id]);
-                const v3 d = mk(rays.a[3][id], rays.a[4][id], rays.a[5][id]);
-                r.rcp = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
-                r.D = mk(1.f / r.rcp.x, 1.f / r.rcp.y, 1.f / r.rcp.z);
-                r.t = rays.a[6][id];
-                r.best = kMiss;
-                r.tests = 0;
-                s.guard = s.s1 = s.s2 = s.s3 = 0;
-                bool ok;
-                if (KIND == 0) ok = (r.t > 0.f) && traverse_init<false>(p, r, s.tEntry, s.cell, s.tMax);
-                else ok = traverse_init<true>(p, r, s.tEntry, s.cell, s.tMax) && (r.t > 0.f);
-                if (ok) has = true;
-                else store_ray<STATS>(g, r, s, id, prim, tout, stats);
-            }
-        }
-        if (__ballot(has) == 0) break;
-        if (has && !top_iteration<KIND>(g, p, r, s, cap_steps)) {
-            store_ray<STATS>(g, r, s, id, prim, tout, stats);
-            has = false;
-        }
-    }
-}
-
-static uint32_t trace_chunk() {
-    static uint32_t c = 0;
-    if (c == 0) {
-        const char* e = getenv("SHINOBI_TRACE_CHUNK");
-        c = e ? (uint32_t)atoi(e) : 256u;
-        if (c < 64) c = 64;
-        c = (c + 63) / 64 * 64;
-    }
-    return c;
-}
-
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim, float* t,
-                  uint32_t* stats, hipStream_t st) {
+                  uint32_t* stats, bool shadow, hipStream_t st) {
     if (n == 0) return;
     // one ray per lane keeps the 64 lanes of a wave on neighbouring rays, so
-    // coherent rays share cell, leaf and triangle loads; the pooled variant
-    // (SHINOBI_TRACE_POOL=1) refills idle lanes but loses that sharing
-    const char* pool = getenv("SHINOBI_TRACE_POOL");
-    if (!(pool && pool[0] == '1')) {
-        const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
-        if (g.kind == 0) {
-            if (stats) hipLaunchKernelGGL((k_trace<0, true>), grid, block, 0, st, g, r, n, d_n, prim, t, stats);
-            else hipLaunchKernelGGL((k_trace<0, false>), grid, block, 0, st, g, r, n, d_n, prim, t, stats);
-        } else {
-            if (stats) hipLaunchKernelGGL((k_trace<1, true>), grid, block, 0, st, g, r, n, d_n, prim, t, stats);
-            else hipLaunchKernelGGL((k_trace<1, false>), grid, block, 0, st, g, r, n, d_n, prim, t, stats);
-        }
-        return;
-    }
-    const uint32_t chunk = trace_chunk();
-    const uint32_t waves = (n + chunk - 1) / chunk;
-    const dim3 grid((waves + kBlock / 64 - 1) / (kBlock / 64)), block(kBlock);
+    // coherent rays share cell, leaf and triangle loads
+    const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
+#define SG_TRACE(K, S, H) hipLaunchKernelGGL((k_trace<K, S, H>), grid, block, 0, st, g, r, n, d_n, prim, t, stats)
     if (g.kind == 0) {
-        if (stats) hipLaunchKernelGGL((k_trace_pool<0, true>), grid, block, 0, st, g, r, n, d_n, prim, t, stats, chunk);
-        else hipLaunchKernelGGL((k_trace_pool<0, false>), grid, block, 0, st, g, r, n, d_n, prim, t, stats, chunk);
+        if (shadow) { if (stats) SG_TRACE(0, true, true); else SG_TRACE(0, false, true); }
+        else { if (stats) SG_TRACE(0, true, false); else SG_TRACE(0, false, false); }
     } else {
-        if (stats) hipLaunchKernelGGL((k_trace_pool<1, true>), grid, block, 0, st, g, r, n, d_n, prim, t, stats, chunk);
-        else hipLaunchKernelGGL((k_trace_pool<1, false>), grid, block, 0, st, g, r, n, d_n, prim, t, stats, chunk);
+        if (shadow) { if (stats) SG_TRACE(1, true, true); else SG_TRACE(1, false, true); }
+        else { if (stats) SG_TRACE(1, true, false); else SG_TRACE(1, false, false); }
     }
+#undef SG_TRACE
 }
 
 // ------------------------------------------------------------------ bounce

@@ -159,3 +159,41 @@ def test_special_rays(ctx, oracle):
         np.testing.assert_array_equal(prim, po)
         np.testing.assert_array_equal(t.view(np.uint32), to.view(np.uint32))
         assert (prim[n // 10: n // 10 + 40] == sa.MISS).all()
+
+
+def _shadow_segments(prim, t, rays, light):
+    hit = prim != sa.MISS
+    o = rays[:3, hit] + rays[3:6, hit] * (t[hit] - np.float32(1e-4))
+    seg = np.empty((7, o.shape[1]), np.float32)
+    seg[:3] = o
+    seg[3:6] = np.asarray(light, np.float32)[:, None] - o
+    seg[6] = 1.0
+    return seg
+
+
+@pytest.mark.parametrize("which", ["soup10k", "heightfield"])
+def test_shadow_any_hit(ctx, oracle, which):
+    """sg_trace_shadow == oracle any-hit traversal, bit for bit, on segments
+    from the primary hits to a point light; occlusion agrees with the
+    closest hit along each segment."""
+    if which == "soup10k":
+        w = W.soup10k()
+        og = oracle.build_uniform(w.scene, **w.oracle)
+        light = (0.5, 0.5, -2.0)
+    else:
+        w = W.heightfield2m(grid=256, res_x=480, res_y=270)
+        og = oracle.build_twolevel(w.scene, **w.oracle)
+        light = (-0.5, 0.5, 0.35)
+    _, g = _build(ctx, w)
+    prim, t = g.trace_host(w.rays)
+    seg = _shadow_segments(prim, t, w.rays, light)
+    assert seg.shape[1] > 1000
+    ps, ts, ss = g.trace_host(seg, with_stats=True, shadow=True)
+    po, to, so = og.trace(seg, with_stats=True, shadow=True)
+    np.testing.assert_array_equal(ps, po)
+    np.testing.assert_array_equal(ts.view(np.uint32), to.view(np.uint32))
+    np.testing.assert_array_equal(ss, so)
+    pc, tc = g.trace_host(seg)
+    np.testing.assert_array_equal(ts < 0.9999, tc < 0.9999)
+    occ = float((ts < 0.9999).mean())
+    assert 0.01 < occ < 0.99, occ

@@ -129,3 +129,48 @@ def test_bounce_rays(oracle, scene):
     assert np.allclose(off, 0.001, atol=2e-5)
     # cosine weighting: E[cos theta] = 2/3
     assert abs(np.mean(np.sum(n * d, axis=0)) - 2.0 / 3.0) < 0.03
+
+
+def _shadow_rays(oracle, g, scene, rays, light):
+    """segments from the closest hits of `rays` towards a point light:
+    origin = hit point, direction = light - origin (unnormalised), tmax = 1"""
+    prim, t = g.trace(rays)
+    hit = prim != MISS
+    o = rays[:3, hit] + rays[3:6, hit] * (t[hit] - 1e-4)
+    d = np.asarray(light, np.float32)[:, None] - o
+    out = np.empty((7, o.shape[1]), np.float32)
+    out[:3], out[3:6], out[6] = o, d, 1.0
+    return out
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_shadow_any_hit(oracle, scene, rays, kind):
+    """any-hit traversal (TwoLevelGridShadowTraverser,
+    src/Control/Algorithm/TwoLevelGridTraverser.h:677-828;
+    UniformGridShadowTraverser, src/Control/Algorithm/SimpleGridTraverser.h:409-464):
+    it visits every cell the closest-hit walk visits, so a segment is
+    occluded (t < 0.9999) exactly when the closest hit along it is closer
+    than 0.9999; a reported occluder is a real intersection that is never
+    closer than the closest one."""
+    g = (oracle.build_twolevel(scene, lambda_leaf=2.0) if kind else oracle.build_uniform(scene, top_divisor=1))
+    sh = _shadow_rays(oracle, g, scene, rays, (0.3, 1.7, 0.9))
+    assert sh.shape[1] > 1000
+    pa, ta = g.trace(sh, shadow=True)
+    pc, tc = g.trace(sh)
+    occ_any = ta < 0.9999
+    occ_closest = tc < 0.9999
+    assert np.array_equal(occ_any, occ_closest)
+    assert 0.05 < occ_any.mean() < 0.95
+    hit = pa != MISS
+    assert (ta[hit] >= tc[hit]).all()
+    # the reported triangle really is hit at the reported t (FP64 judge on that triangle alone)
+    idx = np.flatnonzero(hit)[:400]
+    for i in idx:
+        tri = scene[:, pa[i]:pa[i] + 1]
+        p64, t64 = oracle.brute_f64(np.ascontiguousarray(tri), np.ascontiguousarray(sh[:, i:i + 1]))
+        assert p64[0] == 0 and abs(t64[0] - ta[i]) <= 1e-4 * max(1e-3, abs(t64[0]))
+    # rays that start occluded-short (t < 0.9999) are not traversed at all
+    short = sh.copy()
+    short[6] = 0.5
+    ps, ts = g.trace(short, shadow=True)
+    assert (ps == MISS).all() and (ts == 0.5).all()
