@@ -63,6 +63,8 @@ typedef struct {
     int radix_bits;           /* 4, 6 or 8 (0 = 8)                             */
     float capacity_slack;     /* buffer headroom for rebuilds (0 = 1.25)       */
     int keep_debug;           /* keep per-triangle / per-pair counts           */
+    int trace_block;          /* threads per block of this grid's traversal
+                                 launches: 64, 128 or 256 (0 = 256)           */
 } sg_build_params;
 
 void sg_build_params_default(sg_build_params *p, int kind);
@@ -120,7 +122,7 @@ sg_status sg_device_count(int *out);
 /* Primitive storage: replaces FaceSoup::upload (src/Control/Structure/
  * FaceSoup.h:171-247).  soa9 = v0.x,v0.y,v0.z,v1.x,...,v2.z, n floats each,
  * in host memory (src_on_device = 0) or device memory (1).  The library keeps
- * a device copy and a 48-byte traversal record per triangle. */
+ * a device copy and a 64-byte record per triangle (rewritten by every build). */
 sg_status sg_scene_create(sg_ctx *ctx, const float *const soa9[9], uint32_t n, int src_on_device,
                           void *stream, sg_scene **out);
 /* new vertex positions, same triangle count: replaces FaceSoup::uploadVertices
@@ -205,6 +207,105 @@ sg_status sg_event_create(sg_ctx *ctx, void **out);
 sg_status sg_event_destroy(sg_ctx *ctx, void *ev);
 sg_status sg_event_record(sg_ctx *ctx, void *ev, void *stream);
 sg_status sg_event_elapsed_ms(sg_ctx *ctx, void *ev_start, void *ev_end, float *ms);
+
+/* ------------------------------------------------------------ batching --- */
+/* Many (grid, ray batch) items traced by ONE persistent launch: the
+ * reference's persistent trace kernel (src/Engine/RayTracingKernels.h:37-136,
+ * per-warp pool of ray ids from a global counter, :76) generalised from one
+ * ray buffer to a table of batches against different grids.  Every wave
+ * takes the next 64 rays of the concatenated batches; grids of both kinds mix
+ * in one launch.  Results equal sg_trace_closest / sg_trace_shadow per item. */
+typedef struct sg_batch sg_batch;
+sg_status sg_batch_create(sg_ctx *ctx, uint32_t max_items, sg_batch **out);
+sg_status sg_batch_destroy(sg_batch *batch);
+sg_status sg_batch_clear(sg_batch *batch);
+/* append one item: device SoA rays, n rays, device outputs (prim, t) */
+sg_status sg_batch_add(sg_batch *batch, sg_grid *grid, const sg_rays *rays, uint32_t n, uint32_t *d_prim,
+                       float *d_t);
+/* upload the item table (stream-ordered; the host side may be changed after) */
+sg_status sg_batch_commit(sg_batch *batch, void *stream);
+/* trace every committed item, closest hit (shadow = 0) or any hit (1).
+ * Enqueues kernels only, so it can be captured with sg_graph_begin/end. */
+sg_status sg_batch_launch(sg_batch *batch, int shadow, void *stream);
+
+/* --------------------------------------------------------------- graphs --- */
+/* Stream capture into a replayable HIP graph: rebuilds (sg_rebuild),
+ * single traces and sg_batch_launch enqueue kernels only and may be captured
+ * (sg_build may not: it sizes the grid on the host). */
+typedef struct sg_graph sg_graph;
+sg_status sg_graph_begin(sg_ctx *ctx, void *stream);
+sg_status sg_graph_end(sg_ctx *ctx, void *stream, sg_graph **out);
+sg_status sg_graph_launch(sg_graph *graph, void *stream);
+sg_status sg_graph_destroy(sg_graph *graph);
+
+/* ------------------------------------------------------------ autotuner --- */
+/* Picks grid density, sort radix width and traversal block size for one
+ * scene by measurement: each candidate is built (rebuild timed on the
+ * device) and traces a ray sample; the cost is build_ms + trace_ms scaled to
+ * n_rays.  Coordinate search: density (kind in kind_mask: bit 0 uniform
+ * lambda in {1,2,3,5,8}, bit 1 two-level lambda in {3,5,8} x leaf lambda in
+ * {1.2,2,3}), then radix bits {4,6,8}, then block size {64,128,256}.  The
+ * reference fixes lambda by hand (GridBuilder.h:48) and its SAHGridBuilder
+ * searches a cost model (SAHGridBuilder.h:117-202); this measures instead. */
+typedef struct {
+    sg_build_params best;
+    float build_ms, trace_ms, cost_ms;   /* of the chosen parameters          */
+    uint32_t candidates;                 /* parameter sets measured           */
+} sg_tune_result;
+sg_status sg_autotune(sg_ctx *ctx, sg_scene *scene, const sg_rays *sample, uint32_t n_sample, uint64_t n_rays,
+                      int kind_mask, void *stream, sg_tune_result *out);
+
+/* -------------------------------------------------------------- serving --- */
+/* A request loop for ray queries against grids kept resident on the device.
+ * Requests (host SoA rays in, host prim/t out) are admitted while free slots
+ * and HBM budget (device bytes for request rays and results) allow.
+ * SG_SERVE_BATCHED: continuous batching -- every request that is admitted
+ * while earlier waves run joins the next wave; a wave is one persistent
+ * multi-batch launch (sg_batch_*) with its copies, up to max_waves waves are
+ * in flight on separate streams, so copies and traversal of different waves
+ * overlap.  SG_SERVE_PLAIN: the per-request path of the reference -- one
+ * request at a time: copy in, one trace launch, copy out, synchronise.
+ * Results are identical in both modes and to sg_trace_closest. */
+enum { SG_SERVE_BATCHED = 0, SG_SERVE_PLAIN = 1 };
+typedef struct {
+    int mode;                /* SG_SERVE_BATCHED or SG_SERVE_PLAIN              */
+    size_t hbm_budget;       /* device bytes for in-flight request data        */
+    uint32_t max_requests;   /* requests admitted at once (slots)              */
+    uint32_t max_waves;      /* waves in flight (streams), batched mode        */
+    uint32_t max_wave_rays;  /* rays per wave (0: no cap)                       */
+} sg_server_config;
+typedef struct {
+    uint64_t requests, rays, waves;
+    double p50_us, p99_us, mean_us, max_us;   /* submit -> results in host memory */
+    double wall_s;                            /* first submit -> last completion  */
+    double mrays_per_s;                       /* rays / wall_s                    */
+} sg_server_stats;
+typedef struct sg_server sg_server;
+sg_status sg_server_create(sg_ctx *ctx, const sg_server_config *cfg, sg_server **out);
+/* finishes every admitted request, then stops */
+sg_status sg_server_destroy(sg_server *server);
+/* a resident grid (built by the caller; it must outlive the server) */
+sg_status sg_server_add_grid(sg_server *server, sg_grid *grid, uint32_t *scene_id);
+/* a dynamic scene: the server keeps two copies of the mesh and two grids and
+ * rebuilds the back copy on its own stream (sg_server_update_dynamic) while
+ * waves keep running on the front one (double buffering,
+ * DynamicRTEngine::renderFrame, src/Engine/DynamicRTEngine.h:73-158, with
+ * FastTwoLevelGridBuilder::rebuild at fixed resolution).  update_dynamic
+ * returns once the new grid is in front: requests submitted after it see the
+ * new mesh, requests admitted before it finish on the old one. */
+sg_status sg_server_add_dynamic(sg_server *server, const float *const soa9[9], uint32_t n,
+                                const sg_build_params *params, uint32_t *scene_id);
+sg_status sg_server_update_dynamic(sg_server *server, uint32_t scene_id, const float *const soa9[9]);
+/* host rays7 (7 arrays of n floats) in; prim/t written to host memory when
+ * the request completes.  SG_ERR_BUDGET if it can never fit the budget. */
+sg_status sg_server_submit(sg_server *server, uint32_t scene_id, const float *const rays7[7], uint32_t n,
+                           uint32_t *prim_out, float *t_out, uint64_t *request_id);
+/* wait for one request; latency_us (optional) = submit -> completion */
+sg_status sg_server_wait(sg_server *server, uint64_t request_id, double *latency_us);
+/* wait for every submitted request */
+sg_status sg_server_drain(sg_server *server);
+sg_status sg_server_get_stats(sg_server *server, sg_server_stats *out);
+sg_status sg_server_reset_stats(sg_server *server);
 
 /* ---------------------------------------------------------- generators --- */
 /* Seeded synthetic inputs (host memory), see shinobi_amd/csrc/sg_gen.cpp.

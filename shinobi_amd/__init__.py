@@ -415,6 +415,190 @@ def sort_pairs(ctx: Context, keys: np.ndarray, vals: np.ndarray, key_bits: int =
     return dk.download(np.uint32, keys.size), dv.download(np.uint32, keys.size)
 
 
+# ------------------------------------------------------------------ batching
+class Batch:
+    """Many (grid, ray batch) items traced by one persistent launch
+    (sg_batch_*)."""
+
+    def __init__(self, ctx: Context, max_items: int):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        check(ctx._l.sg_batch_create(ctx.h, max_items, C.byref(self.h)), "sg_batch_create")
+        self._keep: list = []
+
+    def clear(self) -> None:
+        check(self.ctx._l.sg_batch_clear(self.h), "sg_batch_clear")
+        self._keep = []
+
+    def add(self, grid: "Grid", rays: RayBatch, prim: DeviceBuffer, t: DeviceBuffer, n: int | None = None) -> None:
+        r = rays.struct()
+        check(self.ctx._l.sg_batch_add(self.h, grid.h, C.byref(r), rays.n if n is None else n, prim.ptr, t.ptr),
+              "sg_batch_add")
+        self._keep += [grid, rays, prim, t]
+
+    def commit(self, stream=None) -> None:
+        check(self.ctx._l.sg_batch_commit(self.h, self.ctx.stream if stream is None else stream), "sg_batch_commit")
+
+    def launch(self, shadow: bool = False, stream=None) -> None:
+        check(self.ctx._l.sg_batch_launch(self.h, int(shadow), self.ctx.stream if stream is None else stream),
+              "sg_batch_launch")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.ctx._l.sg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A captured, replayable sequence of library calls on the context's
+    stream (sg_graph_*): `with Graph.capture(ctx) as g: ...; g.launch()`."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+
+    @classmethod
+    def capture(cls, ctx: Context) -> "Graph":
+        return cls(ctx)
+
+    def __enter__(self) -> "Graph":
+        check(self.ctx._l.sg_graph_begin(self.ctx.h, self.ctx.stream), "sg_graph_begin")
+        return self
+
+    def __exit__(self, et, ev, tb):
+        st = self.ctx._l.sg_graph_end(self.ctx.h, self.ctx.stream, C.byref(self.h))
+        if et is None:
+            check(st, "sg_graph_end")
+
+    def launch(self) -> None:
+        check(self.ctx._l.sg_graph_launch(self.h, self.ctx.stream), "sg_graph_launch")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.ctx._l.sg_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(ctx: Context, scene: FaceSoup, sample: RayBatch, n_rays: int, kinds: int = 3) -> L.TuneResult:
+    """Measure candidate (density, radix bits, block size) settings for one
+    scene and return the cheapest (sg_autotune); kinds bit 0 = uniform,
+    bit 1 = two-level."""
+    out = L.TuneResult()
+    r = sample.struct()
+    check(ctx._l.sg_autotune(ctx.h, scene.h, C.byref(r), sample.n, int(n_rays), kinds, ctx.stream, C.byref(out)),
+          "sg_autotune")
+    return out
+
+
+# ------------------------------------------------------------------- serving
+@dataclass
+class ServerStats:
+    requests: int
+    rays: int
+    waves: int
+    p50_us: float
+    p99_us: float
+    mean_us: float
+    max_us: float
+    wall_s: float
+    mrays_per_s: float
+
+
+class Server:
+    """Request loop for ray queries against resident grids (sg_server_*).
+    mode: "batched" (continuous batching, multi-batch launches) or "plain"
+    (one request per launch, synchronous)."""
+
+    def __init__(self, ctx: Context, mode: str = "batched", hbm_budget: int = 1 << 30, max_requests: int = 256,
+                 max_waves: int = 2, max_wave_rays: int = 0):
+        self.ctx = ctx
+        cfg = L.ServerConfig()
+        cfg.mode = L.SG_SERVE_PLAIN if mode == "plain" else L.SG_SERVE_BATCHED
+        cfg.hbm_budget = int(hbm_budget)
+        cfg.max_requests = int(max_requests)
+        cfg.max_waves = int(max_waves)
+        cfg.max_wave_rays = int(max_wave_rays)
+        self.h = C.c_void_p()
+        check(ctx._l.sg_server_create(ctx.h, C.byref(cfg), C.byref(self.h)), "sg_server_create")
+        self._keep: dict[int, tuple] = {}
+        self._grids: list = []
+
+    def add_grid(self, grid: "Grid") -> int:
+        sid = C.c_uint32()
+        check(self.ctx._l.sg_server_add_grid(self.h, grid.h, C.byref(sid)), "sg_server_add_grid")
+        self._grids.append(grid)
+        return int(sid.value)
+
+    def add_dynamic(self, soa9: np.ndarray, **params) -> int:
+        soa9 = np.ascontiguousarray(soa9, np.float32)
+        p = build_params(params.pop("kind", L.SG_GRID_TWO_LEVEL), **params)
+        sid = C.c_uint32()
+        check(self.ctx._l.sg_server_add_dynamic(self.h, L.Float9(*[soa9[i].ctypes.data for i in range(9)]),
+                                                soa9.shape[1], C.byref(p), C.byref(sid)), "sg_server_add_dynamic")
+        return int(sid.value)
+
+    def update_dynamic(self, scene_id: int, soa9: np.ndarray) -> None:
+        soa9 = np.ascontiguousarray(soa9, np.float32)
+        check(self.ctx._l.sg_server_update_dynamic(self.h, scene_id,
+                                                   L.Float9(*[soa9[i].ctypes.data for i in range(9)])),
+              "sg_server_update_dynamic")
+
+    def submit(self, scene_id: int, rays7: np.ndarray, prim: np.ndarray | None = None,
+               t: np.ndarray | None = None) -> int:
+        rays7 = np.ascontiguousarray(rays7, np.float32)
+        n = rays7.shape[1]
+        prim = np.empty(n, np.uint32) if prim is None else prim
+        t = np.empty(n, np.float32) if t is None else t
+        rid = C.c_uint64()
+        check(self.ctx._l.sg_server_submit(self.h, scene_id, L.Float7(*[rays7[i].ctypes.data for i in range(7)]), n,
+                                           prim.ctypes.data, t.ctypes.data, C.byref(rid)), "sg_server_submit")
+        self._keep[int(rid.value)] = (rays7, prim, t)
+        return int(rid.value)
+
+    def wait(self, rid: int):
+        """-> (prim, t, latency_us)"""
+        lat = C.c_double()
+        check(self.ctx._l.sg_server_wait(self.h, rid, C.byref(lat)), "sg_server_wait")
+        _, prim, t = self._keep.pop(rid)
+        return prim, t, float(lat.value)
+
+    def drain(self) -> None:
+        check(self.ctx._l.sg_server_drain(self.h), "sg_server_drain")
+        self._keep.clear()
+
+    def stats(self) -> ServerStats:
+        st = L.ServerStats()
+        check(self.ctx._l.sg_server_get_stats(self.h, C.byref(st)), "sg_server_get_stats")
+        return ServerStats(*[getattr(st, f) for f, _ in L.ServerStats._fields_])
+
+    def reset_stats(self) -> None:
+        check(self.ctx._l.sg_server_reset_stats(self.h), "sg_server_reset_stats")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.ctx._l.sg_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ----------------------------------------------------------------- generators
 def _soa(n: int, rows: int):
     a = np.empty((rows, n), np.float32)

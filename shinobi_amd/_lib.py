@@ -43,6 +43,7 @@ class BuildParams(C.Structure):
         ("radix_bits", C.c_int),
         ("capacity_slack", C.c_float),
         ("keep_debug", C.c_int),
+        ("trace_block", C.c_int),
     ]
 
 
@@ -66,6 +67,26 @@ class GridInfo(C.Structure):
 
 class Rays(C.Structure):
     _fields_ = [("o", C.c_void_p * 3), ("d", C.c_void_p * 3), ("tmax", C.c_void_p)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("best", BuildParams), ("build_ms", C.c_float), ("trace_ms", C.c_float), ("cost_ms", C.c_float),
+                ("candidates", C.c_uint32)]
+
+
+SG_SERVE_BATCHED = 0
+SG_SERVE_PLAIN = 1
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("mode", C.c_int), ("hbm_budget", C.c_size_t), ("max_requests", C.c_uint32),
+                ("max_waves", C.c_uint32), ("max_wave_rays", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("rays", C.c_uint64), ("waves", C.c_uint64),
+                ("p50_us", C.c_double), ("p99_us", C.c_double), ("mean_us", C.c_double), ("max_us", C.c_double),
+                ("wall_s", C.c_double), ("mrays_per_s", C.c_double)]
 
 
 # name -> (restype, argtypes); the authoritative list of exported entry points
@@ -107,6 +128,30 @@ SIGNATURES = {
     "sg_event_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sg_event_record": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "sg_event_elapsed_ms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
+    "sg_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "sg_batch_destroy": (C.c_int, [C.c_void_p]),
+    "sg_batch_clear": (C.c_int, [C.c_void_p]),
+    "sg_batch_add": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_uint32, C.c_void_p, C.c_void_p]),
+    "sg_batch_commit": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sg_batch_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "sg_graph_begin": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sg_graph_end": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "sg_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sg_graph_destroy": (C.c_int, [C.c_void_p]),
+    "sg_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_uint32, C.c_uint64, C.c_int,
+                              C.c_void_p, C.POINTER(TuneResult)]),
+    "sg_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "sg_server_destroy": (C.c_int, [C.c_void_p]),
+    "sg_server_add_grid": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)]),
+    "sg_server_add_dynamic": (C.c_int, [C.c_void_p, Float9, C.c_uint32, C.POINTER(BuildParams),
+                                        C.POINTER(C.c_uint32)]),
+    "sg_server_update_dynamic": (C.c_int, [C.c_void_p, C.c_uint32, Float9]),
+    "sg_server_submit": (C.c_int, [C.c_void_p, C.c_uint32, Float7, C.c_uint32, C.c_void_p, C.c_void_p,
+                                   C.POINTER(C.c_uint64)]),
+    "sg_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double)]),
+    "sg_server_drain": (C.c_int, [C.c_void_p]),
+    "sg_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "sg_server_reset_stats": (C.c_int, [C.c_void_p]),
     "sg_gen_soup": (C.c_int, [C.c_uint64, C.c_uint32, C.c_float, Float9]),
     "sg_gen_heightfield": (C.c_int, [C.c_uint64, C.c_uint32, C.c_float, Float9]),
     "sg_heightfield_height": (C.c_float, [C.c_uint64, C.c_float, C.c_float, C.c_float]),

@@ -7,62 +7,7 @@
 #include <mutex>
 #include <vector>
 
-#include "../../include/shinobi_grid.h"
-#include "sg_kernels.hpp"
-
-using namespace sg;
-
-struct sg_ctx {
-    int device;
-    // scratch for the standalone primitives (sg_scan_u32, sg_sort_pairs, sg_bounce)
-    void* scratch = nullptr;
-    size_t scratch_bytes = 0;
-};
-
-struct sg_scene {
-    sg_ctx* ctx;
-    uint32_t n;
-    size_t stride = 0;        // floats between the SoA arrays (multiple of 64: 256-byte aligned arrays)
-    float* soa = nullptr;     // 9*stride floats
-    float4* rec = nullptr;    // 4*n float4 per-triangle records
-};
-
-struct sg_grid {
-    sg_ctx* ctx;
-    sg_scene* scene;
-    sg_build_params p;
-    int kind;
-    uint32_t n_tris;
-    int res[3];
-    // device memory: one allocation carved into the arrays below
-    char* pool = nullptr;
-    size_t pool_bytes = 0;
-    GridHeader* hdr = nullptr;
-    uint32_t cap_C1 = 0, cap_P1 = 0, cap_C2 = 0, cap_P2 = 0;
-    uint32_t *k1[2] = {nullptr, nullptr}, *v1[2] = {nullptr, nullptr};
-    uint32_t *ranges1 = nullptr, *cells1 = nullptr;
-    uint32_t *k2[2] = {nullptr, nullptr}, *v2[2] = {nullptr, nullptr};
-    uint32_t* leaves = nullptr;
-    uint32_t *dbg_tri = nullptr, *dbg_leafc = nullptr, *dbg_pair = nullptr;
-    uint32_t *sums_top = nullptr, *sums_cells = nullptr, *sums_leaf = nullptr;   // per-tile sums
-    uint32_t* bpart = nullptr;                                                   // bounds partials
-    uint32_t *sort1 = nullptr, *sort2 = nullptr;
-    size_t sort1_words = 0, sort2_words = 0;
-    int key_bits1 = 1, key_bits2 = 1, passes1 = 1, passes2 = 1, rb = 8;
-    hipStream_t stream = nullptr;
-};
-
-#define HIPCHK(x)                                   \
-    do {                                            \
-        hipError_t e_ = (x);                        \
-        if (e_ != hipSuccess) return hip_status(e_); \
-    } while (0)
-
-static sg_status hip_status(hipError_t e) {
-    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return SG_ERR_OUT_OF_MEMORY;
-    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return SG_ERR_NO_DEVICE;
-    return SG_ERR_HIP;
-}
+#include "sg_internal.hpp"
 
 static sg_status status_from_bits(uint32_t b) {
     if (b & ST_INVALID) return SG_ERR_INVALID_SCENE;
@@ -111,6 +56,7 @@ void sg_build_params_default(sg_build_params* p, int kind) {
     p->leaf_min_refs = 16;
     p->radix_bits = 8;
     p->capacity_slack = 1.25f;
+    p->trace_block = 256;
 }
 
 sg_status sg_device_count(int* out) {
@@ -375,6 +321,8 @@ sg_status sg_build(sg_ctx* ctx, sg_scene* scene, const sg_build_params* p, void*
     if (p->kind != SG_GRID_UNIFORM && p->kind != SG_GRID_TWO_LEVEL) return SG_ERR_INVALID_ARG;
     if (p->radix_bits != 0 && p->radix_bits != 4 && p->radix_bits != 6 && p->radix_bits != 8)
         return SG_ERR_INVALID_ARG;
+    if (p->trace_block != 0 && p->trace_block != 64 && p->trace_block != 128 && p->trace_block != 256)
+        return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = (hipStream_t)stream;
     sg_grid* g = new sg_grid();
@@ -585,29 +533,6 @@ sg_status sg_grid_destroy(sg_grid* g) {
 }
 
 // ---------------------------------------------------------------- traversal
-static GridView grid_view(const sg_grid* g) {
-    GridView v;
-    const bool two = g->kind == SG_GRID_TWO_LEVEL;
-    const int f1 = g->passes1 & 1, f2 = 0;   // leaf pairs always end in buffer 0
-    v.hdr = g->hdr;
-    v.cells = two ? g->cells1 : g->ranges1;
-    v.leaves = two ? g->leaves : nullptr;
-    v.indices = two ? g->v2[f2] : g->v1[f1];
-    v.rec = g->scene->rec;
-    v.kind = g->kind;
-    return v;
-}
-
-static RaySoA ray_soa(const sg_rays* r) {
-    RaySoA s;
-    for (int i = 0; i < 3; ++i) {
-        s.a[i] = r->o[i];
-        s.a[3 + i] = r->d[i];
-    }
-    s.a[6] = r->tmax;
-    return s;
-}
-
 static sg_status trace_common(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint32_t n, const uint32_t* d_count,
                               uint32_t* d_prim, float* d_t, uint32_t* d_stats, bool shadow, void* stream) {
     if (!ctx || !g || !rays || !d_prim || !d_t) return SG_ERR_INVALID_ARG;
@@ -615,7 +540,8 @@ static sg_status trace_common(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint
         if (!rays->o[i] || !rays->d[i]) return SG_ERR_INVALID_ARG;
     if (!rays->tmax) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
-    launch_trace(grid_view(g), ray_soa(rays), n, d_count, d_prim, d_t, d_stats, shadow, (hipStream_t)stream);
+    launch_trace(grid_view(g), ray_soa(rays), n, d_count, d_prim, d_t, d_stats, shadow, g->p.trace_block,
+                 (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return SG_OK;
 }

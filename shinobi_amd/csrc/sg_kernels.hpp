@@ -105,9 +105,25 @@ void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, 
 
 // ---------------------------------------------------------------- traversal
 void launch_prep_tris(const TriSoA& t, uint32_t n, hipStream_t st);
+// one (grid, ray batch) item of a multi-batch launch; chunk0 = first 64-ray
+// chunk of the item in the concatenation of all items (items sorted by it)
+struct TraceItem {
+    GridView g;
+    RaySoA rays;
+    uint32_t* prim;
+    float* t;
+    uint32_t n;
+    uint32_t chunk0;
+};
+// persistent traversal of n_items items (device table) in one launch; the
+// counter (one device word) is reset by a kernel first, so the pair of
+// launches can be captured in a graph and replayed
+void launch_trace_items(const TraceItem* items, uint32_t n_items, uint32_t total_chunks, uint32_t* counter,
+                        bool shadow, hipStream_t st);
 // closest hit, or with shadow the any-hit traversal that stops at t < 0.9999
+// (block: threads per block, 64, 128 or 256)
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim,
-                  float* t, uint32_t* stats, bool shadow, hipStream_t st);
+                  float* t, uint32_t* stats, bool shadow, int block, hipStream_t st);
 // one cosine-weighted bounce per hit, compacted in primary order; *d_count
 // receives the number of bounce rays.  scratch: bounce_scratch_bytes(n).
 size_t bounce_scratch_bytes(uint32_t n);

@@ -1,0 +1,802 @@
+// sg_serve.hip -- multi-batch launches, graph capture, the autotuner and the
+// serving loop of the C ABI (include/shinobi_grid.h), host side.
+//
+// The reference traces one ray buffer per persistent launch
+// (src/Engine/RayTracingKernels.h:37-136) and its engines call it once per
+// frame.  Here a table of (grid, ray batch) items is traced by one persistent
+// launch (k_trace_items), which is what both the batched multi-scene runs and
+// the serving loop build on.
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "sg_internal.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+uint32_t chunks_of(uint32_t n) { return (n + 63u) / 64u; }
+
+}  // namespace
+
+// ================================================================ batching
+struct sg_batch {
+    sg_ctx* ctx = nullptr;
+    uint32_t cap = 0;
+    std::vector<TraceItem> items;
+    uint32_t next_chunk = 0;
+    TraceItem* pinned = nullptr;   // staging of the table for the upload
+    TraceItem* dev = nullptr;      // device table
+    uint32_t* counter = nullptr;   // device chunk counter
+    hipEvent_t uploaded = nullptr; // the last upload out of `pinned`
+    uint32_t dev_items = 0, dev_chunks = 0;
+};
+
+extern "C" {
+
+sg_status sg_batch_create(sg_ctx* ctx, uint32_t max_items, sg_batch** out) {
+    if (!ctx || !out || max_items == 0) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    sg_batch* b = new sg_batch();
+    b->ctx = ctx;
+    b->cap = max_items;
+    b->items.reserve(max_items);
+    if (hipHostMalloc((void**)&b->pinned, sizeof(TraceItem) * max_items, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&b->dev, sizeof(TraceItem) * max_items) != hipSuccess ||
+        hipMalloc((void**)&b->counter, 256) != hipSuccess || hipEventCreate(&b->uploaded) != hipSuccess) {
+        if (b->pinned) hipHostFree(b->pinned);
+        if (b->dev) hipFree(b->dev);
+        if (b->counter) hipFree(b->counter);
+        delete b;
+        return SG_ERR_OUT_OF_MEMORY;
+    }
+    *out = b;
+    return SG_OK;
+}
+
+sg_status sg_batch_destroy(sg_batch* b) {
+    if (!b) return SG_ERR_INVALID_ARG;
+    hipSetDevice(b->ctx->device);
+    hipEventSynchronize(b->uploaded);
+    hipEventDestroy(b->uploaded);
+    hipHostFree(b->pinned);
+    hipFree(b->dev);
+    hipFree(b->counter);
+    delete b;
+    return SG_OK;
+}
+
+sg_status sg_batch_clear(sg_batch* b) {
+    if (!b) return SG_ERR_INVALID_ARG;
+    b->items.clear();
+    b->next_chunk = 0;
+    return SG_OK;
+}
+
+sg_status sg_batch_add(sg_batch* b, sg_grid* g, const sg_rays* rays, uint32_t n, uint32_t* d_prim, float* d_t) {
+    if (!b || !g || !rays || !d_prim || !d_t) return SG_ERR_INVALID_ARG;
+    for (int i = 0; i < 3; ++i)
+        if (!rays->o[i] || !rays->d[i]) return SG_ERR_INVALID_ARG;
+    if (!rays->tmax || b->items.size() >= b->cap) return SG_ERR_INVALID_ARG;
+    if ((uint64_t)b->next_chunk + chunks_of(n) >= (1ull << 32)) return SG_ERR_INVALID_ARG;
+    TraceItem it;
+    it.g = grid_view(g);
+    it.rays = ray_soa(rays);
+    it.prim = d_prim;
+    it.t = d_t;
+    it.n = n;
+    it.chunk0 = b->next_chunk;
+    b->next_chunk += chunks_of(n);
+    b->items.push_back(it);
+    return SG_OK;
+}
+
+sg_status sg_batch_commit(sg_batch* b, void* stream) {
+    if (!b) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipEventSynchronize(b->uploaded));   // the staging buffer is free again
+    const size_t bytes = sizeof(TraceItem) * b->items.size();
+    if (bytes) {
+        memcpy(b->pinned, b->items.data(), bytes);
+        HIPCHK(hipMemcpyAsync(b->dev, b->pinned, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    }
+    HIPCHK(hipEventRecord(b->uploaded, (hipStream_t)stream));
+    b->dev_items = (uint32_t)b->items.size();
+    b->dev_chunks = b->next_chunk;
+    return SG_OK;
+}
+
+sg_status sg_batch_launch(sg_batch* b, int shadow, void* stream) {
+    if (!b) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    launch_trace_items(b->dev, b->dev_items, b->dev_chunks, b->counter, shadow != 0, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return SG_OK;
+}
+
+}  // extern "C"
+
+// ================================================================== graphs
+struct sg_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+sg_status sg_graph_begin(sg_ctx* ctx, void* stream) {
+    if (!ctx || !stream) return SG_ERR_INVALID_ARG;   // the null stream cannot be captured
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return SG_OK;
+}
+
+sg_status sg_graph_end(sg_ctx* ctx, void* stream, sg_graph** out) {
+    if (!ctx || !stream || !out) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    sg_graph* g = new sg_graph();
+    hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g->graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (g->graph) hipGraphDestroy(g->graph);
+        delete g;
+        return hip_status(e);
+    }
+    *out = g;
+    return SG_OK;
+}
+
+sg_status sg_graph_launch(sg_graph* g, void* stream) {
+    if (!g) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return SG_OK;
+}
+
+sg_status sg_graph_destroy(sg_graph* g) {
+    if (!g) return SG_ERR_INVALID_ARG;
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
+    return SG_OK;
+}
+
+// =============================================================== autotuner
+sg_status sg_autotune(sg_ctx* ctx, sg_scene* scene, const sg_rays* sample, uint32_t n_sample, uint64_t n_rays,
+                      int kind_mask, void* stream, sg_tune_result* out) {
+    if (!ctx || !scene || !sample || !out || n_sample == 0 || (kind_mask & 3) == 0) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    uint32_t* d_prim = nullptr;
+    float* d_t = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipMalloc((void**)&d_prim, 4ull * n_sample) != hipSuccess ||
+        hipMalloc((void**)&d_t, 4ull * n_sample) != hipSuccess) {
+        hipFree(d_prim);
+        return SG_ERR_OUT_OF_MEMORY;
+    }
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const double scale = (double)n_rays / (double)n_sample;
+    uint32_t measured = 0;
+
+    // build once (sizing), then the best of three timed rebuilds and traces
+    auto measure = [&](const sg_build_params& p, float* bms, float* tms) -> bool {
+        sg_grid* g = nullptr;
+        if (sg_build(ctx, scene, &p, stream, &g) != SG_OK) return false;
+        sg_grid_info info;
+        if (sg_grid_get_info(g, &info) != SG_OK || info.status != SG_OK) {
+            sg_grid_destroy(g);
+            return false;
+        }
+        float best_b = 1e30f, best_t = 1e30f, ms = 0.f;
+        for (int r = 0; r < 3; ++r) {
+            hipEventRecord(e0, st);
+            sg_rebuild(ctx, scene, g, stream);
+            hipEventRecord(e1, st);
+            hipEventSynchronize(e1);
+            hipEventElapsedTime(&ms, e0, e1);
+            best_b = std::min(best_b, ms);
+            hipEventRecord(e0, st);
+            sg_trace_closest(ctx, g, sample, n_sample, nullptr, d_prim, d_t, nullptr, stream);
+            hipEventRecord(e1, st);
+            hipEventSynchronize(e1);
+            hipEventElapsedTime(&ms, e0, e1);
+            best_t = std::min(best_t, ms);
+        }
+        sg_grid_destroy(g);
+        ++measured;
+        *bms = best_b;
+        *tms = best_t;
+        return true;
+    };
+
+    bool have = false;
+    sg_build_params best;
+    float best_b = 0.f, best_t = 0.f;
+    double best_cost = 1e300;
+    auto consider = [&](const sg_build_params& p) {
+        float b, t;
+        if (!measure(p, &b, &t)) return;
+        const double cost = b + t * scale;
+        if (!have || cost < best_cost) {
+            have = true;
+            best = p;
+            best_b = b;
+            best_t = t;
+            best_cost = cost;
+        }
+    };
+    // 1. density (radix 8, 256 threads)
+    if (kind_mask & 1) {
+        for (float lam : {1.f, 2.f, 3.f, 5.f, 8.f}) {
+            sg_build_params p;
+            sg_build_params_default(&p, SG_GRID_UNIFORM);
+            p.lambda = lam;
+            consider(p);
+        }
+    }
+    if (kind_mask & 2) {
+        for (float lam : {3.f, 5.f, 8.f})
+            for (float ll : {1.2f, 2.f, 3.f}) {
+                sg_build_params p;
+                sg_build_params_default(&p, SG_GRID_TWO_LEVEL);
+                p.lambda = lam;
+                p.lambda_leaf = ll;
+                consider(p);
+            }
+    }
+    // 2. sort radix width, 3. traversal block size, each from the best so far
+    if (have) {
+        const sg_build_params base = best;
+        for (int rb : {4, 6}) {
+            sg_build_params p = base;
+            p.radix_bits = rb;
+            consider(p);
+        }
+        const sg_build_params base2 = best;
+        for (int blk : {64, 128}) {
+            sg_build_params p = base2;
+            p.trace_block = blk;
+            consider(p);
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(d_prim);
+    hipFree(d_t);
+    if (!have) return SG_ERR_INVALID_SCENE;
+    out->best = best;
+    out->build_ms = best_b;
+    out->trace_ms = best_t;
+    out->cost_ms = (float)best_cost;
+    out->candidates = measured;
+    return SG_OK;
+}
+
+}  // extern "C"
+
+// ================================================================= serving
+namespace {
+
+struct Request {
+    uint64_t id = 0;
+    uint32_t scene = 0;
+    uint32_t n = 0;
+    const float* rays[7] = {};
+    uint32_t* prim_out = nullptr;
+    float* t_out = nullptr;
+    size_t bytes = 0;            // device bytes while in flight
+    int gi = 0;                  // grid of a dynamic scene the request runs on
+    Clock::time_point t_submit, t_done;
+    sg_status status = SG_OK;
+    bool done = false;
+};
+
+struct Slot {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    TraceItem* pinned_items = nullptr;
+    TraceItem* dev_items = nullptr;
+    uint32_t* counter = nullptr;
+    bool busy = false;
+    std::vector<Request*> reqs;
+    size_t off = 0, bytes = 0, rays_bytes = 0;   // arena region of the wave
+};
+
+struct SceneEntry {
+    bool dynamic = false;
+    sg_grid* grid[2] = {nullptr, nullptr};
+    sg_scene* mesh[2] = {nullptr, nullptr};
+    int front = 0;
+    bool swap_pending = false;      // back grid rebuilt on the rebuild stream
+    hipEvent_t rebuilt = nullptr;
+    uint32_t in_use[2] = {0, 0};    // admitted requests that read each grid
+};
+
+// first-fit allocator over [0, size)
+struct Arena {
+    std::map<size_t, size_t> free_;   // offset -> bytes
+    explicit Arena(size_t size = 0) {
+        if (size) free_[0] = size;
+    }
+    bool alloc(size_t bytes, size_t* off) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                const size_t rest = it->second - bytes, o = it->first + bytes;
+                free_.erase(it);
+                if (rest) free_[o] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+size_t request_bytes(uint32_t n) { return align_up((size_t)n * 4, kAlign) * 9; }
+
+}  // namespace
+
+struct sg_server {
+    sg_ctx* ctx = nullptr;
+    sg_server_config cfg{};
+    char* dev = nullptr;       // device arena (hbm_budget bytes)
+    char* host = nullptr;      // pinned mirror of the arena
+    Arena arena;
+    std::vector<Slot> slots;
+    std::vector<SceneEntry> scenes;
+    hipStream_t rebuild_st = nullptr;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> pending;
+    std::map<uint64_t, Request*> live;   // submitted, not yet waited for
+    uint64_t next_id = 1;
+    uint32_t admitted = 0;               // requests in busy slots
+    bool stop = false;
+    std::thread worker;
+
+    // statistics since the last reset
+    std::vector<double> lat_us;
+    uint64_t st_rays = 0, st_waves = 0;
+    bool have_first = false;
+    Clock::time_point first_submit, last_done;
+};
+
+namespace {
+
+// copy the wave's requests into the pinned mirror, upload, trace, download
+sg_status launch_wave(sg_server* s, Slot& sl) {
+    const bool plain = s->cfg.mode == SG_SERVE_PLAIN;
+    char* hbase = s->host + sl.off;
+    char* dbase = s->dev + sl.off;
+    // layout: [rays of every request][prim, t of every request]
+    size_t ro = 0;
+    std::vector<size_t> rofs(sl.reqs.size());
+    for (size_t k = 0; k < sl.reqs.size(); ++k) {
+        rofs[k] = ro;
+        ro += align_up((size_t)sl.reqs[k]->n * 4, kAlign) * 7;
+    }
+    sl.rays_bytes = ro;
+    size_t oo = ro;
+    std::vector<size_t> oofs(sl.reqs.size());
+    for (size_t k = 0; k < sl.reqs.size(); ++k) {
+        oofs[k] = oo;
+        oo += align_up((size_t)sl.reqs[k]->n * 4, kAlign) * 2;
+    }
+    for (size_t k = 0; k < sl.reqs.size(); ++k) {
+        const Request* r = sl.reqs[k];
+        const size_t a = align_up((size_t)r->n * 4, kAlign);
+        for (int c = 0; c < 7; ++c) memcpy(hbase + rofs[k] + c * a, r->rays[c], (size_t)r->n * 4);
+    }
+    HIPCHK(hipMemcpyAsync(dbase, hbase, ro, hipMemcpyHostToDevice, sl.st));
+    uint32_t chunk = 0;
+    for (size_t k = 0; k < sl.reqs.size(); ++k) {
+        const Request* r = sl.reqs[k];
+        SceneEntry& se = s->scenes[r->scene];
+        const int gi = r->gi;
+        const size_t a = align_up((size_t)r->n * 4, kAlign);
+        TraceItem it;
+        it.g = grid_view(se.grid[gi]);
+        for (int c = 0; c < 7; ++c) it.rays.a[c] = reinterpret_cast<const float*>(dbase + rofs[k] + c * a);
+        it.prim = reinterpret_cast<uint32_t*>(dbase + oofs[k]);
+        it.t = reinterpret_cast<float*>(dbase + oofs[k] + a);
+        it.n = r->n;
+        it.chunk0 = chunk;
+        chunk += chunks_of(r->n);
+        if (plain) {
+            launch_trace(it.g, it.rays, it.n, nullptr, it.prim, it.t, nullptr, false, se.grid[gi]->p.trace_block,
+                         sl.st);
+        } else {
+            sl.pinned_items[k] = it;
+        }
+    }
+    if (!plain) {
+        HIPCHK(hipMemcpyAsync(sl.dev_items, sl.pinned_items, sizeof(TraceItem) * sl.reqs.size(),
+                              hipMemcpyHostToDevice, sl.st));
+        launch_trace_items(sl.dev_items, (uint32_t)sl.reqs.size(), chunk, sl.counter, false, sl.st);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hbase + ro, dbase + ro, oo - ro, hipMemcpyDeviceToHost, sl.st));
+    HIPCHK(hipEventRecord(sl.ev, sl.st));
+    return SG_OK;
+}
+
+// results to the callers' buffers; called without the lock held
+void finish_wave(sg_server* s, Slot& sl) {
+    const char* hbase = s->host + sl.off;
+    size_t oo = sl.rays_bytes;
+    for (Request* r : sl.reqs) {
+        const size_t a = align_up((size_t)r->n * 4, kAlign);
+        memcpy(r->prim_out, hbase + oo, (size_t)r->n * 4);
+        memcpy(r->t_out, hbase + oo + a, (size_t)r->n * 4);
+        oo += 2 * a;
+    }
+}
+
+void serve_loop(sg_server* s) {
+    hipSetDevice(s->ctx->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        bool progress = false;
+        // 1. completed waves
+        for (Slot& sl : s->slots) {
+            if (!sl.busy) continue;
+            if (hipEventQuery(sl.ev) != hipSuccess) continue;
+            lk.unlock();
+            finish_wave(s, sl);
+            const auto now = Clock::now();
+            lk.lock();
+            for (Request* r : sl.reqs) {
+                s->scenes[r->scene].in_use[r->gi]--;
+                r->t_done = now;
+                r->done = true;
+                s->lat_us.push_back(std::chrono::duration<double, std::micro>(now - r->t_submit).count());
+                s->st_rays += r->n;
+            }
+            s->last_done = now;
+            ++s->st_waves;
+            s->arena.release(sl.off, sl.bytes);
+            s->admitted -= (uint32_t)sl.reqs.size();
+            sl.reqs.clear();
+            sl.busy = false;
+            progress = true;
+            s->cv_done.notify_all();
+        }
+        // 2. dynamic scenes whose back grid finished rebuilding
+        for (SceneEntry& se : s->scenes) {
+            if (se.dynamic && se.swap_pending && hipEventQuery(se.rebuilt) == hipSuccess) {
+                se.front = 1 - se.front;
+                se.swap_pending = false;
+                s->cv_done.notify_all();
+            }
+        }
+        // 3. a new wave from the pending requests
+        Slot* free_slot = nullptr;
+        for (Slot& sl : s->slots)
+            if (!sl.busy) {
+                free_slot = &sl;
+                break;
+            }
+        if (free_slot && !s->pending.empty()) {
+            const bool plain = s->cfg.mode == SG_SERVE_PLAIN;
+            size_t bytes = 0;
+            uint64_t rays = 0;
+            size_t take = 0;
+            for (Request* r : s->pending) {
+                if (take > 0 && plain) break;
+                if (s->admitted + take + 1 > s->cfg.max_requests) break;
+                if (take > 0 && s->cfg.max_wave_rays && rays + r->n > s->cfg.max_wave_rays) break;
+                bytes += r->bytes;
+                rays += r->n;
+                ++take;
+            }
+            size_t off = 0;
+            // shrink the wave until its region fits the free arena
+            while (take > 0 && !s->arena.alloc(bytes, &off)) {
+                --take;
+                bytes -= s->pending[take]->bytes;
+            }
+            if (take > 0) {
+                Slot& sl = *free_slot;
+                sl.reqs.assign(s->pending.begin(), s->pending.begin() + take);
+                s->pending.erase(s->pending.begin(), s->pending.begin() + take);
+                for (Request* r : sl.reqs) {
+                    SceneEntry& se = s->scenes[r->scene];
+                    r->gi = se.dynamic ? se.front : 0;
+                    se.in_use[r->gi]++;
+                }
+                sl.off = off;
+                sl.bytes = bytes;
+                sl.busy = true;
+                s->admitted += (uint32_t)take;
+                lk.unlock();
+                const sg_status e = launch_wave(s, sl);
+                lk.lock();
+                if (e != SG_OK)
+                    for (Request* r : sl.reqs) r->status = e;
+                progress = true;
+            }
+        }
+        bool busy = false;
+        for (Slot& sl : s->slots) busy |= sl.busy;
+        for (const SceneEntry& se : s->scenes) busy |= se.swap_pending;
+        if (s->stop && s->pending.empty() && !busy) break;
+        if (!progress) {
+            if (busy || (s->pending.size() && !free_slot)) {
+                // the device is working: poll its events at a short interval
+                s->cv_work.wait_for(lk, std::chrono::microseconds(20));
+            } else {
+                s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+sg_status sg_server_create(sg_ctx* ctx, const sg_server_config* cfg, sg_server** out) {
+    if (!ctx || !cfg || !out || cfg->hbm_budget < (1u << 20) || cfg->max_requests == 0) return SG_ERR_INVALID_ARG;
+    if (cfg->mode != SG_SERVE_BATCHED && cfg->mode != SG_SERVE_PLAIN) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    sg_server* s = new sg_server();
+    s->ctx = ctx;
+    s->cfg = *cfg;
+    if (s->cfg.mode == SG_SERVE_PLAIN) s->cfg.max_waves = 1;
+    if (s->cfg.max_waves == 0) s->cfg.max_waves = 2;
+    const size_t budget = align_up(cfg->hbm_budget, kAlign);
+    s->cfg.hbm_budget = budget;
+    s->arena = Arena(budget);
+    bool ok = hipMalloc((void**)&s->dev, budget) == hipSuccess &&
+              hipHostMalloc((void**)&s->host, budget, hipHostMallocDefault) == hipSuccess &&
+              hipStreamCreateWithFlags(&s->rebuild_st, hipStreamNonBlocking) == hipSuccess;
+    s->slots.resize(s->cfg.max_waves);
+    for (Slot& sl : s->slots) {
+        ok = ok && hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) == hipSuccess &&
+             hipHostMalloc((void**)&sl.pinned_items, sizeof(TraceItem) * s->cfg.max_requests,
+                           hipHostMallocDefault) == hipSuccess &&
+             hipMalloc((void**)&sl.dev_items, sizeof(TraceItem) * s->cfg.max_requests) == hipSuccess &&
+             hipMalloc((void**)&sl.counter, 256) == hipSuccess;
+    }
+    if (!ok) {
+        s->stop = true;
+        sg_server_destroy(s);
+        return SG_ERR_OUT_OF_MEMORY;
+    }
+    s->worker = std::thread(serve_loop, s);
+    *out = s;
+    return SG_OK;
+}
+
+sg_status sg_server_destroy(sg_server* s) {
+    if (!s) return SG_ERR_INVALID_ARG;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    hipSetDevice(s->ctx->device);
+    if (s->rebuild_st) hipStreamSynchronize(s->rebuild_st);
+    for (Slot& sl : s->slots) {
+        if (sl.st) hipStreamSynchronize(sl.st);
+        if (sl.st) hipStreamDestroy(sl.st);
+        if (sl.ev) hipEventDestroy(sl.ev);
+        if (sl.pinned_items) hipHostFree(sl.pinned_items);
+        if (sl.dev_items) hipFree(sl.dev_items);
+        if (sl.counter) hipFree(sl.counter);
+    }
+    for (SceneEntry& se : s->scenes) {
+        if (!se.dynamic) continue;
+        for (int b = 0; b < 2; ++b) {
+            if (se.grid[b]) sg_grid_destroy(se.grid[b]);
+            if (se.mesh[b]) sg_scene_destroy(se.mesh[b]);
+        }
+        if (se.rebuilt) hipEventDestroy(se.rebuilt);
+    }
+    for (auto& kv : s->live) delete kv.second;
+    for (Request* r : s->pending) delete r;
+    if (s->rebuild_st) hipStreamDestroy(s->rebuild_st);
+    if (s->dev) hipFree(s->dev);
+    if (s->host) hipHostFree(s->host);
+    delete s;
+    return SG_OK;
+}
+
+sg_status sg_server_add_grid(sg_server* s, sg_grid* g, uint32_t* id) {
+    if (!s || !g || !id) return SG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    SceneEntry se;
+    se.grid[0] = g;
+    s->scenes.push_back(se);
+    *id = (uint32_t)s->scenes.size() - 1;
+    return SG_OK;
+}
+
+sg_status sg_server_add_dynamic(sg_server* s, const float* const soa9[9], uint32_t n, const sg_build_params* p,
+                                uint32_t* id) {
+    if (!s || !soa9 || !p || !id || n == 0) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(s->ctx->device));
+    SceneEntry se;
+    se.dynamic = true;
+    sg_status r = SG_OK;
+    for (int b = 0; b < 2 && r == SG_OK; ++b) {
+        r = sg_scene_create(s->ctx, soa9, n, 0, s->rebuild_st, &se.mesh[b]);
+        if (r == SG_OK) r = sg_build(s->ctx, se.mesh[b], p, s->rebuild_st, &se.grid[b]);
+    }
+    if (r == SG_OK && hipEventCreateWithFlags(&se.rebuilt, hipEventDisableTiming) != hipSuccess) r = SG_ERR_HIP;
+    if (r == SG_OK && hipStreamSynchronize(s->rebuild_st) != hipSuccess) r = SG_ERR_HIP;
+    if (r != SG_OK) {
+        for (int b = 0; b < 2; ++b) {
+            if (se.grid[b]) sg_grid_destroy(se.grid[b]);
+            if (se.mesh[b]) sg_scene_destroy(se.mesh[b]);
+        }
+        return r;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->scenes.push_back(se);
+    *id = (uint32_t)s->scenes.size() - 1;
+    return SG_OK;
+}
+
+sg_status sg_server_update_dynamic(sg_server* s, uint32_t id, const float* const soa9[9]) {
+    if (!s || !soa9) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(s->ctx->device));
+    int back;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (id >= s->scenes.size() || !s->scenes[id].dynamic) return SG_ERR_INVALID_ARG;
+        // one rebuild at a time (the previous one has been swapped in), and
+        // the requests still reading the back grid finish before it is
+        // overwritten; new requests only take the front grid
+        s->cv_done.wait(lk, [&] {
+            const SceneEntry& se = s->scenes[id];
+            return !se.swap_pending && se.in_use[1 - se.front] == 0;
+        });
+        back = 1 - s->scenes[id].front;
+    }
+    SceneEntry& se = s->scenes[id];
+    sg_status r = sg_scene_update(se.mesh[back], soa9, 0, s->rebuild_st);
+    if (r == SG_OK) r = sg_rebuild(s->ctx, se.mesh[back], se.grid[back], s->rebuild_st);
+    if (r != SG_OK) return r;
+    HIPCHK(hipEventRecord(se.rebuilt, s->rebuild_st));
+    std::unique_lock<std::mutex> lk(s->mu);
+    se.swap_pending = true;
+    s->cv_work.notify_all();
+    // returns once requests admitted from now on see the new mesh
+    s->cv_done.wait(lk, [&] { return !s->scenes[id].swap_pending; });
+    return SG_OK;
+}
+
+sg_status sg_server_submit(sg_server* s, uint32_t scene, const float* const rays7[7], uint32_t n,
+                           uint32_t* prim_out, float* t_out, uint64_t* id) {
+    if (!s || !rays7 || !prim_out || !t_out || !id || n == 0) return SG_ERR_INVALID_ARG;
+    for (int c = 0; c < 7; ++c)
+        if (!rays7[c]) return SG_ERR_INVALID_ARG;
+    const size_t bytes = request_bytes(n);
+    if (bytes > s->cfg.hbm_budget) return SG_ERR_BUDGET;
+    Request* r = new Request();
+    r->scene = scene;
+    r->n = n;
+    for (int c = 0; c < 7; ++c) r->rays[c] = rays7[c];
+    r->prim_out = prim_out;
+    r->t_out = t_out;
+    r->bytes = bytes;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (scene >= s->scenes.size()) {
+            delete r;
+            return SG_ERR_INVALID_ARG;
+        }
+        r->id = s->next_id++;
+        r->t_submit = Clock::now();
+        if (!s->have_first) {
+            s->have_first = true;
+            s->first_submit = r->t_submit;
+        }
+        s->pending.push_back(r);
+        s->live[r->id] = r;
+        *id = r->id;
+    }
+    s->cv_work.notify_all();
+    return SG_OK;
+}
+
+sg_status sg_server_wait(sg_server* s, uint64_t id, double* latency_us) {
+    if (!s) return SG_ERR_INVALID_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->live.find(id);
+    if (it == s->live.end()) return SG_ERR_INVALID_ARG;
+    Request* r = it->second;
+    s->cv_done.wait(lk, [&] { return r->done; });
+    if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(r->t_done - r->t_submit).count();
+    const sg_status st = r->status;
+    s->live.erase(it);
+    delete r;
+    return st;
+}
+
+sg_status sg_server_drain(sg_server* s) {
+    if (!s) return SG_ERR_INVALID_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] {
+        for (auto& kv : s->live)
+            if (!kv.second->done) return false;
+        return true;
+    });
+    sg_status st = SG_OK;
+    for (auto& kv : s->live) {
+        if (kv.second->status != SG_OK) st = kv.second->status;
+        delete kv.second;
+    }
+    s->live.clear();
+    return st;
+}
+
+sg_status sg_server_get_stats(sg_server* s, sg_server_stats* out) {
+    if (!s || !out) return SG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    memset(out, 0, sizeof(*out));
+    std::vector<double> v = s->lat_us;
+    out->requests = v.size();
+    out->rays = s->st_rays;
+    out->waves = s->st_waves;
+    if (!v.empty()) {
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double q) {
+            const size_t k = (size_t)std::min<double>(v.size() - 1, std::max(0.0, q * (double)v.size() - 1e-9));
+            return v[k];
+        };
+        out->p50_us = pct(0.50);
+        out->p99_us = pct(0.99);
+        double sum = 0;
+        for (double x : v) sum += x;
+        out->mean_us = sum / v.size();
+        out->max_us = v.back();
+        out->wall_s = std::chrono::duration<double>(s->last_done - s->first_submit).count();
+        out->mrays_per_s = out->wall_s > 0 ? (double)s->st_rays / out->wall_s / 1e6 : 0.0;
+    }
+    return SG_OK;
+}
+
+sg_status sg_server_reset_stats(sg_server* s) {
+    if (!s) return SG_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->lat_us.clear();
+    s->st_rays = 0;
+    s->st_waves = 0;
+    s->have_first = false;
+    return SG_OK;
+}
+
+}  // extern "C"

@@ -9,6 +9,8 @@
 // intersector would perform, so results are unchanged and every
 // Moller-Trumbore test needs two 16-byte loads plus one 4-byte load from one
 // aligned record after the index fetch.
+#include <algorithm>
+
 #include "sg_device.hpp"
 #include "sg_kernels.hpp"
 
@@ -229,15 +231,11 @@ SG_D void trace_twolevel(const GridView& g, const GParams& p, Ray& r, uint32_t& 
     }
 }
 
-// the trace kernel of src/Engine/RayTracingKernels.h:37-136 for SoA rays:
+// the trace kernel of src/Engine/RayTracingKernels.h:37-136 for one SoA ray:
 // rcp(dir) (:111-113), traverse, store prim = indices[best] (miss: ~0u) and t
 template <int KIND, bool STATS, bool SHADOW>
-__global__ __launch_bounds__(kBlock) void k_trace(GridView g, RaySoA rays, uint32_t cap, const uint32_t* d_n,
-                                                  uint32_t* __restrict__ prim, float* __restrict__ tout,
-                                                  uint32_t* __restrict__ stats) {
-    const uint32_t n = d_n ? min(*d_n, cap) : cap;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+SG_D void trace_one(const GridView& g, const RaySoA& rays, uint32_t i, uint32_t* __restrict__ prim,
+                    float* __restrict__ tout, uint32_t* __restrict__ stats) {
     // a failed or overflowed build is never traversed: every ray misses
     if (g.hdr->status & ~(ST_LEAF_RES | ST_NEG_EXTENT)) {
         prim[i] = kMiss;
@@ -268,13 +266,25 @@ __global__ __launch_bounds__(kBlock) void k_trace(GridView g, RaySoA rays, uint3
     }
 }
 
+// one ray per lane over one batch; the block size is a launch parameter
+template <int KIND, bool STATS, bool SHADOW>
+__global__ __launch_bounds__(kBlock) void k_trace(GridView g, RaySoA rays, uint32_t cap, const uint32_t* d_n,
+                                                  uint32_t* __restrict__ prim, float* __restrict__ tout,
+                                                  uint32_t* __restrict__ stats) {
+    const uint32_t n = d_n ? min(*d_n, cap) : cap;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    trace_one<KIND, STATS, SHADOW>(g, rays, i, prim, tout, stats);
+}
+
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim, float* t,
-                  uint32_t* stats, bool shadow, hipStream_t st) {
+                  uint32_t* stats, bool shadow, int block, hipStream_t st) {
     if (n == 0) return;
     // one ray per lane keeps the 64 lanes of a wave on neighbouring rays, so
     // coherent rays share cell, leaf and triangle loads
-    const dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
-#define SG_TRACE(K, S, H) hipLaunchKernelGGL((k_trace<K, S, H>), grid, block, 0, st, g, r, n, d_n, prim, t, stats)
+    const int B = (block == 64 || block == 128 || block == 256) ? block : kBlock;
+    const dim3 grid((n + B - 1) / B), tb(B);
+#define SG_TRACE(K, S, H) hipLaunchKernelGGL((k_trace<K, S, H>), grid, tb, 0, st, g, r, n, d_n, prim, t, stats)
     if (g.kind == 0) {
         if (shadow) { if (stats) SG_TRACE(0, true, true); else SG_TRACE(0, false, true); }
         else { if (stats) SG_TRACE(0, true, false); else SG_TRACE(0, false, false); }
@@ -283,6 +293,68 @@ void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t
         else { if (stats) SG_TRACE(1, true, false); else SG_TRACE(1, false, false); }
     }
 #undef SG_TRACE
+}
+
+// ------------------------------------------------- many batches, one launch
+// Persistent traversal of a table of (grid, ray batch) items: every wave
+// takes the next 64-ray chunk of the concatenated batches from one counter
+// (one device atomic per 64 rays, far below the atomic unit's rate) and
+// traces it against that chunk's grid, so batches of any size and grids of
+// either kind share one launch and the load balances across the chip.
+__global__ __launch_bounds__(kBlock) void k_trace_items(const TraceItem* __restrict__ items, uint32_t n_items,
+                                                        uint32_t total_chunks, uint32_t* __restrict__ counter,
+                                                        int shadow) {
+    const int l = lane_id();
+    for (;;) {
+        uint32_t c = 0;
+        if (l == 0) c = atomicAdd(counter, 1u);
+        c = __builtin_amdgcn_readfirstlane(c);
+        if (c >= total_chunks) return;
+        // last item whose first chunk is <= c
+        uint32_t lo = 0, hi = n_items;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (items[mid].chunk0 <= c) lo = mid;
+            else hi = mid;
+        }
+        const TraceItem& it = items[lo];
+        const uint32_t i = (c - it.chunk0) * 64u + (uint32_t)l;
+        if (i < it.n) {
+            if (it.g.kind == 0) {
+                if (shadow) trace_one<0, false, true>(it.g, it.rays, i, it.prim, it.t, nullptr);
+                else trace_one<0, false, false>(it.g, it.rays, i, it.prim, it.t, nullptr);
+            } else {
+                if (shadow) trace_one<1, false, true>(it.g, it.rays, i, it.prim, it.t, nullptr);
+                else trace_one<1, false, false>(it.g, it.rays, i, it.prim, it.t, nullptr);
+            }
+        }
+    }
+}
+
+__global__ void k_set_counter(uint32_t* p) {
+    if (threadIdx.x == 0) *p = 0u;
+}
+
+static int g_item_blocks = 0;
+
+void launch_trace_items(const TraceItem* items, uint32_t n_items, uint32_t total_chunks, uint32_t* counter,
+                        bool shadow, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, st, counter);
+    if (n_items == 0 || total_chunks == 0) return;
+    if (g_item_blocks == 0) {
+        int dev = 0, cus = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_trace_items),
+                                                            kBlock, 0);
+        g_item_blocks = std::max(1, cus) * std::max(1, per);
+    }
+    const uint32_t waves_needed = total_chunks;
+    uint32_t blocks = (uint32_t)g_item_blocks;
+    const uint32_t max_blocks = (waves_needed + kBlock / 64 - 1) / (kBlock / 64);
+    if (blocks > max_blocks) blocks = max_blocks;
+    hipLaunchKernelGGL(k_trace_items, dim3(blocks), dim3(kBlock), 0, st, items, n_items, total_chunks, counter,
+                       shadow ? 1 : 0);
 }
 
 // ------------------------------------------------------------------ bounce

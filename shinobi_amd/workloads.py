@@ -64,3 +64,28 @@ def cluster(n_small: int = 500_000, n_large: int = 2_000, res_x: int = 1280, res
     return Workload("cluster", SG_GRID_TWO_LEVEL, scene, rays,
                     dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16), bounce_seed=seed + 100,
                     oracle=dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16))
+
+
+def sphere_mesh(n_tris: int = 200_000, frame: int = 0, seed: int = 5) -> np.ndarray:
+    """Latitude/longitude tessellation of a sphere (centre (0.5,0.5,0.5),
+    radius 0.3) with about n_tris triangles, its radius displaced by a seeded
+    sine wave that moves with `frame` -- the dynamic mesh of the serving
+    workload.  Returns (9, N) float32 triangle SoA."""
+    rng = np.random.default_rng(seed)
+    k = rng.uniform(4.0, 9.0, 3)
+    ph = rng.uniform(0.0, 2.0 * np.pi, 3)
+    n_lat = max(4, int(np.sqrt(n_tris / 4.0)))
+    n_lon = max(8, n_tris // (2 * n_lat))
+    th = np.linspace(0.0, np.pi, n_lat + 1)
+    phi = np.linspace(0.0, 2.0 * np.pi, n_lon + 1)
+    T, P = np.meshgrid(th, phi, indexing="ij")
+    d = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)])
+    w = 0.25 * frame
+    r = 0.3 + 0.02 * (np.sin(k[0] * d[0] + ph[0] + w) * np.sin(k[1] * d[1] + ph[1] + 0.7 * w)
+                      + 0.5 * np.sin(k[2] * d[2] + ph[2] - w))
+    v = (0.5 + r * d).astype(np.float32)            # (3, n_lat+1, n_lon+1)
+    a, b = v[:, :-1, :-1], v[:, 1:, :-1]
+    c, e = v[:, 1:, 1:], v[:, :-1, 1:]
+    t1 = np.concatenate([a, b, c]).reshape(9, -1)
+    t2 = np.concatenate([a, c, e]).reshape(9, -1)
+    return np.ascontiguousarray(np.concatenate([t1, t2], axis=1), np.float32)
