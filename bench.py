@@ -84,6 +84,184 @@ def cpu_baseline(w, n_primary: int, seed: int, threads: int) -> dict:
                       f"primary and {b.shape[1]} bounce rays, {threads} threads"}
 
 
+def _log_uniform(rng, lo, hi, n):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), n)).astype(np.int64)
+
+
+def serving_bench(sa, W, ctx, seconds: float = 2.0, rate: float = 2000.0, burst: int = 400) -> dict:
+    """Served ray queries: 16 resident static scenes (50k-2M triangles,
+    uniform soups and height fields) plus a 200k-triangle sphere rebuilt every
+    33 ms on the server's own stream; Poisson arrivals at `rate` req/s, each a
+    batch of log-uniform [1e3, 1e6] random rays against a random scene, host
+    data in pinned memory.  The same arrival schedule runs against the plain
+    per-request path and the continuous-batching server; then a burst of
+    `burst` requests submitted at once measures each path's throughput at
+    saturation."""
+    import threading
+    rng = np.random.default_rng(5)
+    sizes = _log_uniform(rng, 50_000, 2_000_000, 16)
+    grids = []
+    for k, nt in enumerate(sizes):
+        if k % 2 == 0:
+            s9 = sa.gen_soup(500 + k, int(nt), 0.01 * (1e4 / nt) ** (1.0 / 3.0))
+        else:
+            g = max(8, int(np.sqrt(nt / 2.0)) + 1)
+            s9 = sa.gen_heightfield(500 + k, g, 0.2)
+        soup = sa.FaceSoup(ctx, s9)
+        b = sa.FastTwoLevelGridBuilder(ctx, lam=5.0, lambda_leaf=2.0)
+        b.init(soup)
+        grids.append((soup, b.build()))
+    n_tris = [int(g.scene.n) for _, g in grids]
+    # request templates in pinned memory (a request is a (7, n) block)
+    K = 48
+    tsz = _log_uniform(rng, 1_000, 1_000_000, K)
+    temps = []
+    for k, n in enumerate(tsz):
+        r = sa.PinnedArray(ctx, (7, int(n)), np.float32)
+        r.a[:] = sa.gen_random_rays(900 + k, int(n), 0.0, 1.0)
+        outs = [(sa.PinnedArray(ctx, (int(n),), np.uint32), sa.PinnedArray(ctx, (int(n),), np.float32))
+                for _ in range(2)]
+        temps.append((r, outs))
+    frames = [W.sphere_mesh(200_000, frame=f) for f in range(4)]
+    n_req = int(seconds * rate)
+    gaps = rng.exponential(1.0 / rate, n_req)
+    which = rng.integers(0, K, n_req)
+    scene_of = rng.integers(0, 17, n_req)
+
+    def run(mode: str, timed: bool):
+        srv = sa.Server(ctx, mode=mode, hbm_budget=3 << 30, max_requests=512, max_waves=3)
+        ids = [srv.add_grid(g) for _, g in grids]
+        ids.append(srv.add_dynamic(frames[0], lam=5.0, lambda_leaf=2.0))
+        stop = threading.Event()
+
+        def producer():
+            f = 0
+            while not stop.is_set():
+                t0 = time.perf_counter()
+                f += 1
+                srv.update_dynamic(ids[16], frames[f % len(frames)])
+                stop.wait(max(0.0, 0.033 - (time.perf_counter() - t0)))
+
+        th = threading.Thread(target=producer, daemon=True)
+        th.start()
+        srv.reset_stats()
+        t = time.perf_counter()
+        n_sub = n_req if timed else burst
+        for k in range(n_sub):
+            if timed:
+                t += gaps[k]
+                while time.perf_counter() < t:
+                    pass
+            r, outs = temps[which[k]]
+            o = outs[k & 1]
+            srv.submit(ids[scene_of[k]], r.a, o[0].a, o[1].a)
+        srv.drain()
+        st = srv.stats()
+        stop.set()
+        th.join()
+        srv.close()
+        return {"requests": st.requests, "p50_us": round(st.p50_us, 1), "p99_us": round(st.p99_us, 1),
+                "mean_us": round(st.mean_us, 1), "mrays_per_s": round(st.mrays_per_s, 1), "waves": st.waves}
+
+    out = {"scenes": {"static_tris": n_tris, "dynamic_tris": int(frames[0].shape[1]), "rebuild_ms_period": 33},
+           "offered_req_per_s": rate,
+           "offered_mrays_per_s": round(rate * float(np.mean(tsz[which])) / 1e6, 1),
+           "mean_rays_per_request": round(float(np.mean(tsz[which])), 1)}
+    out["plain"] = run("plain", True)
+    out["batched"] = run("batched", True)
+    sat_p = run("plain", False)
+    sat_b = run("batched", False)
+    out["saturated"] = {"requests": burst, "plain": sat_p, "batched": sat_b,
+                        "throughput_ratio": round(sat_b["mrays_per_s"] / max(sat_p["mrays_per_s"], 1e-9), 2),
+                        "p99_ratio": round(sat_b["p99_us"] / max(sat_p["p99_us"], 1e-9), 3)}
+    return out
+
+
+def batched_bench(sa, ctx, n_scenes: int = 64, reps: int = 5) -> dict:
+    """Batched multi-scene launch: 64 uniform-in-cube soups, N log-uniform in
+    [1e3, 5e5] with edge scale 0.01*N^(-1/3), each with log-uniform
+    [1e4, 2e6] random rays; per-scene density, radix width and block size
+    from the autotuner.  One captured graph rebuilds every grid and traces
+    every batch in one persistent launch; the plain path runs the same
+    rebuilds and one trace launch per scene."""
+    rng = np.random.default_rng(4)
+    ns = _log_uniform(rng, 1_000, 500_000, n_scenes)
+    nr = _log_uniform(rng, 10_000, 2_000_000, n_scenes)
+    items = []
+    t_tune = time.perf_counter()
+    kinds = {0: 0, 1: 0}
+    radix = {4: 0, 6: 0, 8: 0}
+    blocks = {64: 0, 128: 0, 256: 0}
+    for k in range(n_scenes):
+        s9 = sa.gen_soup(4000 + k, int(ns[k]), float(0.01 * ns[k] ** (-1.0 / 3.0)))
+        soup = sa.FaceSoup(ctx, s9)
+        rays = sa.RayBatch(ctx, sa.gen_random_rays(6000 + k, int(nr[k]), 0.0, 1.0))
+        sample = sa.RayBatch(ctx, sa.gen_random_rays(8000 + k, min(20_000, int(nr[k])), 0.0, 1.0))
+        tr = sa.autotune(ctx, soup, sample, int(nr[k]))
+        p = tr.best
+        kinds[p.kind] += 1
+        radix[p.radix_bits] += 1
+        blocks[p.trace_block] += 1
+        B = sa.FastTwoLevelGridBuilder if p.kind == 1 else sa.FastSimpleGridBuilder
+        b = B(ctx, lam=p.lambda_, lambda_leaf=p.lambda_leaf, radix_bits=p.radix_bits, trace_block=p.trace_block)
+        b.init(soup)
+        g = b.build()
+        prim, tb = sa.DeviceBuffer(ctx, 4 * rays.n), sa.DeviceBuffer(ctx, 4 * rays.n)
+        items.append((soup, g, rays, prim, tb))
+    t_tune = time.perf_counter() - t_tune
+    batch = sa.Batch(ctx, n_scenes)
+    for soup, g, rays, prim, tb in items:
+        batch.add(g, rays, prim, tb)
+    batch.commit()
+    ctx.sync()
+    with sa.Graph.capture(ctx) as graph:
+        for soup, g, rays, prim, tb in items:
+            g.rebuild()
+        batch.launch()
+    e0, e1 = ctx.event(), ctx.event()
+
+    def timeit(fn):
+        fn()
+        ctx.sync()
+        ts = []
+        for _ in range(reps):
+            e0.record()
+            fn()
+            e1.record()
+            ctx.sync()
+            ts.append(e0.elapsed_ms(e1))
+        return float(np.median(ts))
+
+    def plain():
+        for soup, g, rays, prim, tb in items:
+            g.rebuild()
+            g.trace(rays, prim=prim, t=tb)
+
+    def plain_trace():
+        for soup, g, rays, prim, tb in items:
+            g.trace(rays, prim=prim, t=tb)
+
+    ms_graph = timeit(graph.launch)
+    ms_plain = timeit(plain)
+    ms_batch_trace = timeit(lambda: batch.launch())
+    ms_plain_trace = timeit(plain_trace)
+    # the batched results equal the per-scene launches
+    ref = [(p.download(np.uint32, r.n)) for _, _, r, p, _ in items]
+    graph.launch()
+    ctx.sync()
+    same = all(np.array_equal(p.download(np.uint32, r.n), q) for (_, _, r, p, _), q in zip(items, ref))
+    total_rays = int(nr.sum())
+    graph.close()
+    return {"scenes": n_scenes, "tris_total": int(ns.sum()), "rays_total": total_rays,
+            "autotune_s": round(t_tune, 1), "chosen": {"two_level": kinds[1], "uniform": kinds[0],
+                                                       "radix_bits": radix, "block": blocks},
+            "graph_ms": round(ms_graph, 3), "plain_ms": round(ms_plain, 3),
+            "batched_trace_ms": round(ms_batch_trace, 3), "plain_trace_ms": round(ms_plain_trace, 3),
+            "graph_mrays_per_s": round(total_rays / ms_graph / 1e3, 1),
+            "plain_mrays_per_s": round(total_rays / ms_plain / 1e3, 1),
+            "results_equal": bool(same)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +269,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-serving", action="store_true", help="skip the served-queries measurement")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-scene measurement")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -230,6 +410,10 @@ def main() -> int:
         "roofline_build": roofline_build,
         "cpu_baseline": cpu,
     }
+    if not args.no_batched:
+        out["batched"] = batched_bench(sa, ctx)
+    if not args.no_serving:
+        out["serving"] = serving_bench(sa, W, ctx)
     print(json.dumps(out))
     return 0
 

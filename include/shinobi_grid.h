@@ -198,6 +198,10 @@ sg_status sg_sort_pairs(sg_ctx *ctx, uint32_t *d_keys, uint32_t *d_vals, uint32_
 /* -------------------------------------------------- device plumbing --- */
 sg_status sg_malloc(sg_ctx *ctx, size_t bytes, void **out);
 sg_status sg_free(sg_ctx *ctx, void *p);
+/* pinned host memory (page-locked): copies to and from it are DMA transfers
+ * that overlap device work; serving clients keep request data in it */
+sg_status sg_host_alloc(sg_ctx *ctx, size_t bytes, void **out);
+sg_status sg_host_free(sg_ctx *ctx, void *p);
 sg_status sg_memcpy_htod(sg_ctx *ctx, void *dst, const void *src, size_t bytes, void *stream);
 sg_status sg_memcpy_dtoh(sg_ctx *ctx, void *dst, const void *src, size_t bytes, void *stream);
 sg_status sg_stream_create(sg_ctx *ctx, void **out);
@@ -296,8 +300,11 @@ sg_status sg_server_add_grid(sg_server *server, sg_grid *grid, uint32_t *scene_i
 sg_status sg_server_add_dynamic(sg_server *server, const float *const soa9[9], uint32_t n,
                                 const sg_build_params *params, uint32_t *scene_id);
 sg_status sg_server_update_dynamic(sg_server *server, uint32_t scene_id, const float *const soa9[9]);
-/* host rays7 (7 arrays of n floats) in; prim/t written to host memory when
- * the request completes.  SG_ERR_BUDGET if it can never fit the budget. */
+/* host rays7 (7 arrays of n floats; one contiguous 7 x n block is copied
+ * with one transfer) in; prim/t written to host memory when the request
+ * completes.  The server copies straight from and to these buffers, which
+ * must stay valid until then; pinned buffers (sg_host_alloc) make the copies
+ * asynchronous.  SG_ERR_BUDGET if the request can never fit the budget. */
 sg_status sg_server_submit(sg_server *server, uint32_t scene_id, const float *const rays7[7], uint32_t n,
                            uint32_t *prim_out, float *t_out, uint64_t *request_id);
 /* wait for one request; latency_us (optional) = submit -> completion */

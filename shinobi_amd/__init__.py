@@ -161,6 +161,31 @@ class DeviceBuffer:
             pass
 
 
+class PinnedArray:
+    """A numpy array in pinned host memory (sg_host_alloc); .a is the array."""
+
+    def __init__(self, ctx: Context, shape, dtype=np.float32):
+        self.ctx = ctx
+        dtype = np.dtype(dtype)
+        n = int(np.prod(shape)) * dtype.itemsize
+        self.ptr = C.c_void_p()
+        check(ctx._l.sg_host_alloc(ctx.h, max(n, 1), C.byref(self.ptr)), "sg_host_alloc")
+        buf = (C.c_char * max(n, 1)).from_address(self.ptr.value)
+        self.a = np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def close(self) -> None:
+        if self.ptr:
+            self.a = None
+            self.ctx._l.sg_host_free(self.ctx.h, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # --------------------------------------------------------------------- rays
 class RayBatch:
     """SoA rays (ox,oy,oz,dx,dy,dz,tmax) resident on the device."""

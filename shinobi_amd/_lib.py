@@ -119,6 +119,8 @@ SIGNATURES = {
     "sg_sort_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p]),
     "sg_malloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "sg_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sg_host_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "sg_host_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sg_memcpy_htod": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "sg_memcpy_dtoh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "sg_stream_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),

@@ -210,7 +210,8 @@ static size_t grid_layout(sg_grid* g, char* pool) {
     g->bpart = (uint32_t*)take(bounds_blocks(N) * 6 * 4 + 16);
     g->sums_top = (uint32_t*)take(emit_tiles(N) * 4 + 16);
     g->sums_cells = two ? (uint32_t*)take(cell_tiles(g->cap_C1) * 4 + 16) : nullptr;
-    g->sums_leaf = two ? (uint32_t*)take((2 * leaf_tiles(g->cap_P1) + 2) * 4 + 16) : nullptr;
+    g->sums_leaf = two ? (uint32_t*)take(leaf_sums_words(g->cap_P1) * 4) : nullptr;
+    g->leaf_info = two ? (uint32_t*)take(leaf_info_words(g->cap_P1) * 4) : nullptr;
     g->key_bits1 = bit_length(g->cap_C1);
     g->passes1 = sort_passes(g->key_bits1, g->rb);
     g->sort1_words = sort_scratch_words(g->cap_P1, g->key_bits1, g->rb);
@@ -281,6 +282,8 @@ static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st, void* cons
     seg(g->ranges1, (size_t)g->cap_C1 * 8);
     if (two) {
         seg(g->leaves, (size_t)g->cap_C2 * 8);
+        seg(g->sums_leaf, leaf_sums_words(g->cap_P1) * 4);
+        seg(g->leaf_info, leaf_info_words(g->cap_P1) * 4);
     }
     // per-pair leaf counts are only written below P1_valid
     seg(g->dbg_pair, (size_t)g->cap_P1 * 4);
@@ -301,7 +304,7 @@ static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st, void* cons
     launch_leaf_cells(g->hdr, a, g->ranges1, g->cells1, g->dbg_leafc, g->sums_cells, st);
     mark(5);
     launch_leaf_emit(t, g->hdr, a, g->k1[f1], g->v1[f1], g->ranges1, g->cells1, g->k2[0], g->v2[0], g->dbg_pair,
-                     g->sums_leaf, st);
+                     g->sums_leaf, g->leaf_info, st);
     mark(6);
     launch_leaf_sort_fixup(g->k2[0], g->v2[0], g->k2[1], g->v2[1], g->hdr, g->cap_P2, g->key_bits2, g->rb, g->sort2,
                            st);
@@ -528,11 +531,15 @@ sg_status sg_grid_destroy(sg_grid* g) {
         hipStreamSynchronize(g->stream);
         hipFree(g->pool);
     }
+    if (g->tscratch) hipFree(g->tscratch);
     delete g;
     return SG_OK;
 }
 
 // ---------------------------------------------------------------- traversal
+// batches below this many rays walk in one pass
+constexpr uint32_t kMultiPassMin = 65536;
+
 static sg_status trace_common(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint32_t n, const uint32_t* d_count,
                               uint32_t* d_prim, float* d_t, uint32_t* d_stats, bool shadow, void* stream) {
     if (!ctx || !g || !rays || !d_prim || !d_t) return SG_ERR_INVALID_ARG;
@@ -540,7 +547,21 @@ static sg_status trace_common(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint
         if (!rays->o[i] || !rays->d[i]) return SG_ERR_INVALID_ARG;
     if (!rays->tmax) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
-    launch_trace(grid_view(g), ray_soa(rays), n, d_count, d_prim, d_t, d_stats, shadow, g->p.trace_block,
+    // large batches walk in compacting passes (queues in per-grid scratch;
+    // traces of one grid on different streams must not overlap in time)
+    void* scratch = nullptr;
+    if (!d_stats && n >= kMultiPassMin) {
+        const size_t need = trace_scratch_bytes(n);
+        if (g->tscratch_bytes < need) {
+            if (g->tscratch) HIPCHK(hipFree(g->tscratch));
+            g->tscratch = nullptr;
+            g->tscratch_bytes = 0;
+            if (hipMalloc(&g->tscratch, need) != hipSuccess) return SG_ERR_OUT_OF_MEMORY;
+            g->tscratch_bytes = need;
+        }
+        scratch = g->tscratch;
+    }
+    launch_trace(grid_view(g), ray_soa(rays), n, d_count, d_prim, d_t, d_stats, shadow, g->p.trace_block, scratch,
                  (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return SG_OK;
@@ -633,6 +654,19 @@ sg_status sg_free(sg_ctx* ctx, void* p) {
     if (!ctx) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
     if (p) HIPCHK(hipFree(p));
+    return SG_OK;
+}
+
+sg_status sg_host_alloc(sg_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return SG_ERR_OUT_OF_MEMORY;
+    return SG_OK;
+}
+
+sg_status sg_host_free(sg_ctx* ctx, void* p) {
+    if (!ctx) return SG_ERR_INVALID_ARG;
+    if (p) HIPCHK(hipHostFree(p));
     return SG_OK;
 }
 

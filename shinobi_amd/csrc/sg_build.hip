@@ -48,49 +48,48 @@ SG_D void tri_load(const TriSoA& t, uint32_t i, v3& a, v3& b, v3& c) {
 
 // --------------------------------------------------------------- bounds
 // Scene AABB (the host loop of FastTwoLevelGridBuilder.h:75-81) as a device
-// reduction; exact because min/max never round.  The SoA arrays are 16-byte
-// aligned, so each lane reads four triangles of an array with one load.  The
-// same pass writes the per-triangle records (TriSoA::rec), so they always
-// match the vertices the grid is built from.  Each block leaves its six
-// partial keys in `part`; k_params folds them (no same-address atomics).
+// reduction; exact because min/max never round.  The same pass writes the
+// per-triangle records (TriSoA::rec), so they always match the vertices the
+// grid is built from: a block takes 256 triangles (coalesced SoA loads, one
+// triangle per lane), stages their 4 KB x 4 of records in LDS and stores
+// them with lane-consecutive 16-byte writes (full lines, no partial-line
+// write traffic).  Each block leaves its six partial keys in `part`;
+// k_params folds them (no same-address atomics).
 __global__ __launch_bounds__(kBlock) void k_bounds(TriSoA t, uint32_t n, uint32_t* __restrict__ part) {
     __shared__ uint32_t s_red[6][kBlock / 64];
+    __shared__ float4 s_rec[4 * kBlock];
     uint32_t k[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-    const uint32_t n4 = n / 4;
-    for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < n4; q += gridDim.x * kBlock) {
-        float4 f[9];
+    const uint32_t chunks = (n + kBlock - 1) / kBlock;
+    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const uint32_t base = c * kBlock;
+        const uint32_t i = base + threadIdx.x;
+        if (i < n) {
+            float w[9];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) f[c] = reinterpret_cast<const float4*>(t.v[c])[q];
+            for (int q = 0; q < 9; ++q) w[q] = t.v[q][i];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-#pragma unroll
-            for (int v = 0; v < 3; ++v) {
-                const float4 g = f[3 * v + a];
-                const uint32_t k0 = fkey(g.x), k1 = fkey(g.y), k2 = fkey(g.z), k3 = fkey(g.w);
-                k[a] = min(k[a], min(min(k0, k1), min(k2, k3)));
-                k[3 + a] = max(k[3 + a], max(max(k0, k1), max(k2, k3)));
+            for (int a = 0; a < 3; ++a) {
+                const uint32_t k0 = fkey(w[a]), k1 = fkey(w[3 + a]), k2 = fkey(w[6 + a]);
+                k[a] = min(k[a], min(k0, min(k1, k2)));
+                k[3 + a] = max(k[3 + a], max(k0, max(k1, k2)));
             }
+            const v3 v0 = mk(w[0], w[1], w[2]), v1 = mk(w[3], w[4], w[5]), v2 = mk(w[6], w[7], w[8]);
+            const v3 e1 = v1 - v0, e2 = v2 - v0;
+            // record component j of triangle threadIdx.x at s_rec[j * kBlock + threadIdx.x]
+            s_rec[0 * kBlock + threadIdx.x] = make_float4(v0.x, v0.y, v0.z, e1.x);
+            s_rec[1 * kBlock + threadIdx.x] = make_float4(e1.y, e1.z, e2.x, e2.y);
+            s_rec[2 * kBlock + threadIdx.x] = make_float4(e2.z, v1.x, v1.y, v1.z);
+            s_rec[3 * kBlock + threadIdx.x] = make_float4(v2.x, v2.y, v2.z, 0.f);
         }
-        store_tri_record(t.rec, 4 * q + 0, mk(f[0].x, f[1].x, f[2].x), mk(f[3].x, f[4].x, f[5].x),
-                         mk(f[6].x, f[7].x, f[8].x));
-        store_tri_record(t.rec, 4 * q + 1, mk(f[0].y, f[1].y, f[2].y), mk(f[3].y, f[4].y, f[5].y),
-                         mk(f[6].y, f[7].y, f[8].y));
-        store_tri_record(t.rec, 4 * q + 2, mk(f[0].z, f[1].z, f[2].z), mk(f[3].z, f[4].z, f[5].z),
-                         mk(f[6].z, f[7].z, f[8].z));
-        store_tri_record(t.rec, 4 * q + 3, mk(f[0].w, f[1].w, f[2].w), mk(f[3].w, f[4].w, f[5].w),
-                         mk(f[6].w, f[7].w, f[8].w));
-    }
-    for (uint32_t i = n4 * 4 + blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        v3 v0, v1, v2;
-        tri_load(t, i, v0, v1, v2);
-        store_tri_record(t.rec, i, v0, v1, v2);
-        const float w[9] = {v0.x, v0.y, v0.z, v1.x, v1.y, v1.z, v2.x, v2.y, v2.z};
+        __syncthreads();
+        // float4 number f = 4 * tri + component of the block's 1024 float4s
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const uint32_t k0 = fkey(w[a]), k1 = fkey(w[3 + a]), k2 = fkey(w[6 + a]);
-            k[a] = min(k[a], min(k0, min(k1, k2)));
-            k[3 + a] = max(k[3 + a], max(k0, max(k1, k2)));
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t f = (uint32_t)j * kBlock + threadIdx.x;
+            const uint32_t tri = f >> 2;
+            if (base + tri < n) t.rec[4 * (size_t)base + f] = s_rec[(f & 3u) * kBlock + tri];
         }
+        __syncthreads();
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
@@ -572,98 +571,160 @@ SG_D uint32_t leaf_pair(const TriSoA& t, const LeafParams& L, GridHeader* h, uin
     return (uint32_t)c;
 }
 
-// first top-cell start at or after sorted top pair i (P1v when none)
-SG_D uint32_t top_cell_start(uint32_t i, uint32_t P1v, uint32_t C1, const uint32_t* __restrict__ top_keys,
-                             const uint32_t* __restrict__ ranges) {
-    if (i >= P1v) return P1v;
-    const uint32_t k = top_keys[i];
-    if (k >= C1) return i;
-    const uint2 r = reinterpret_cast<const uint2*>(ranges)[k];
-    if (r.x == i) return i;
-    return (r.y > i && r.y <= P1v) ? r.y : i;
-}
+// Leaf tile of a sorted top pair: tile t holds the top cells that start in
+// [t*kLeafTile, (t+1)*kLeafTile), so pair i belongs to tile
+// floor(start of its top cell / kLeafTile).  Per tile the count kernel
+// records (zero-initialised words, combined with atomicMax):
+//   info[4t+0] = ~first pair   info[4t+1] = end pair
+//   info[4t+2] = ~first leaf key   info[4t+3] = last leaf key + 1
+// and its accepted / rejected leaf-pair totals in sums[t] / sums[Tp + t].
+SG_HD uint32_t leaf_sums_stride(uint32_t T) { return (T + 15u) & ~15u; }
 
-SG_D void leaf_tile(const GridHeader* h, const uint32_t* __restrict__ top_keys, const uint32_t* __restrict__ ranges,
-                    uint32_t t, uint32_t* a0, uint32_t* a1) {
-    const uint32_t P1v = h->P1_valid, C1 = h->C1;
-    *a0 = top_cell_start(t * kLeafTile, P1v, C1, top_keys, ranges);
-    const uint32_t e = top_cell_start((t + 1) * kLeafTile, P1v, C1, top_keys, ranges);
-    *a1 = e > *a0 ? e : *a0;
-}
-
-// sums[t] = accepted, sums[T + t] = rejected leaf pairs of tile t (T tiles)
-__global__ __launch_bounds__(kLeafBlock) void k_leaf_count(TriSoA t, GridHeader* h,
+__global__ __launch_bounds__(kBlock) void k_leaf_count(TriSoA t, GridHeader* h,
                                                        const uint32_t* __restrict__ top_keys,
                                                        const uint32_t* __restrict__ top_vals,
                                                        const uint32_t* __restrict__ ranges,
                                                        const uint32_t* __restrict__ cells,
-                                                       uint32_t* __restrict__ sums, uint32_t T,
-                                                       uint32_t* __restrict__ dbg_counts) {
-    __shared__ uint32_t s_red[kLeafBlock / 64];
-    uint32_t a0, a1;
-    leaf_tile(h, top_keys, ranges, blockIdx.x, &a0, &a1);
-    uint32_t acc = 0, all = 0;
-    if (a0 < a1) {
-        const LeafParams L = read_leaf_params(h);
-        for (uint32_t i = a0 + threadIdx.x; i < a1; i += kLeafBlock) {
-            LeafSetup s;
-            const uint32_t c = leaf_pair(t, L, h, top_keys[i], top_vals[i], cells, s);
-            if (dbg_counts) dbg_counts[i] = c;
-            if (c) {
-                all += c;
-                leaf_visit(L, s, [&](uint32_t, bool ok) { acc += ok ? 1u : 0u; });
+                                                       uint32_t* __restrict__ sums, uint32_t* __restrict__ info,
+                                                       uint32_t T, uint32_t* __restrict__ dbg_counts) {
+    constexpr int kSlots = 4;
+    __shared__ uint32_t s_acc[kSlots], s_rej[kSlots];
+    const uint32_t P1v = h->P1_valid;
+    const uint32_t base = blockIdx.x * kTile;
+    if (base >= P1v) return;
+    if (threadIdx.x < kSlots) s_acc[threadIdx.x] = s_rej[threadIdx.x] = 0;
+    const LeafParams L = read_leaf_params(h);
+    const uint32_t Tp = leaf_sums_stride(T);
+    const uint32_t i0 = base + threadIdx.x * kItems;
+    uint32_t k4[kItems], t4[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) k4[j] = t4[j] = 0;
+    if (i0 + kItems <= P1v) {
+        const uint4 kk = reinterpret_cast<const uint4*>(top_keys)[i0 / 4];
+        const uint4 vv = reinterpret_cast<const uint4*>(top_vals)[i0 / 4];
+        k4[0] = kk.x; k4[1] = kk.y; k4[2] = kk.z; k4[3] = kk.w;
+        t4[0] = vv.x; t4[1] = vv.y; t4[2] = vv.z; t4[3] = vv.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            if (i0 + j < P1v) {
+                k4[j] = top_keys[i0 + j];
+                t4[j] = top_vals[i0 + j];
+            }
+    }
+    // the block's first pair may sit in a top cell that starts far earlier:
+    // slot 0 is that pair's tile, slots 1.. the tiles from base on
+    const uint32_t first_key = top_keys[base];
+    const uint32_t tile0 = first_key < L.C1 ? reinterpret_cast<const uint2*>(ranges)[first_key].x / kLeafTile : 0u;
+    const uint32_t tile1 = base / kLeafTile;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const uint32_t i = i0 + j;
+        if (i >= P1v || k4[j] >= L.C1) continue;
+        const uint2 rg = reinterpret_cast<const uint2*>(ranges)[k4[j]];
+        const uint32_t tile = rg.x / kLeafTile;
+        LeafSetup st;
+        const uint32_t c = leaf_pair(t, L, h, k4[j], t4[j], cells, st);
+        if (dbg_counts) dbg_counts[i] = c;
+        uint32_t acc = 0;
+        if (c) leaf_visit(L, st, [&](uint32_t, bool ok) { acc += ok ? 1u : 0u; });
+        if (tile < T) {
+            const uint32_t slot = tile == tile0 ? 0u : tile - tile1 + 1u;
+            if (slot < kSlots) {
+                if (acc) atomicAdd(&s_acc[slot], acc);
+                if (c - acc) atomicAdd(&s_rej[slot], c - acc);
+            } else {
+                atomicAdd(&sums[tile], acc);
+                atomicAdd(&sums[Tp + tile], c - acc);
+            }
+            if (rg.x == i) {   // first pair of its top cell
+                const uint2 cl = reinterpret_cast<const uint2*>(cells)[k4[j]];
+                const uint32_t nl = (cl.y & 0xFFu) * ((cl.y >> 8) & 0xFFu) * ((cl.y >> 16) & 0xFFu);
+                atomicMax(&info[4 * (size_t)tile + 0], ~i);
+                atomicMax(&info[4 * (size_t)tile + 1], rg.y);
+                atomicMax(&info[4 * (size_t)tile + 2], ~cl.x);
+                atomicMax(&info[4 * (size_t)tile + 3], cl.x + nl);
             }
         }
     }
-    const uint32_t sa = block_sum<kLeafBlock / 64>(acc, s_red);
-    const uint32_t sl = block_sum<kLeafBlock / 64>(all, s_red);
-    if (threadIdx.x == 0) {
-        sums[blockIdx.x] = sa;
-        sums[T + blockIdx.x] = sl - sa;
+    __syncthreads();
+    if (threadIdx.x < kSlots) {
+        const uint32_t slot = threadIdx.x;
+        const uint32_t tile = slot == 0 ? tile0 : tile1 + slot - 1u;
+        if (tile < T && (slot == 0 || tile != tile0)) {
+            if (s_acc[slot]) atomicAdd(&sums[tile], s_acc[slot]);
+            if (s_rej[slot]) atomicAdd(&sums[Tp + tile], s_rej[slot]);
+        }
     }
 }
 
 // Exclusive scan, in place, of the accepted (sums[0..T)) and rejected
-// (sums[T..2T)) tile totals; sums[2T], sums[2T+1] receive the grand totals and
-// the header the leaf-pair counts (one block of 1024 threads).
+// (sums[Tp..Tp+T)) tile totals, both halves at once (waves 0-7 and 8-15);
+// sums[2Tp], sums[2Tp+1] receive the grand totals and the header the
+// leaf-pair counts (one block of 1024 threads).
 __global__ __launch_bounds__(1024) void k_leaf_scan(GridHeader* h, BuildArgs a, uint32_t* __restrict__ sums,
                                                     uint32_t T) {
-    __shared__ uint32_t s_scan[16];
-    uint32_t tot[2];
-    for (int half = 0; half < 2; ++half) {
-        uint32_t* row = sums + (size_t)half * T;
-        uint32_t run = 0;
-        for (uint32_t c0 = 0; c0 < T; c0 += 1024 * 8) {
-            const uint32_t i0 = c0 + threadIdx.x * 8;
-            uint32_t v[8], sum = 0;
+    constexpr int kPer = 16;
+    __shared__ uint32_t s_w[16];
+    __shared__ uint32_t s_tot[2];
+    const uint32_t Tp = leaf_sums_stride(T);
+    const int w = wave_id(), half = w >> 3;
+    const uint32_t tt = threadIdx.x & 511u;
+    uint32_t* row = sums + (size_t)half * Tp;
+    uint32_t run = 0;
+    for (uint32_t c0 = 0; c0 < T; c0 += 512u * kPer) {
+        const uint32_t i0 = c0 + tt * kPer;
+        uint32_t v[kPer], sum = 0;
+        if (i0 + kPer <= Tp) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                v[k] = i0 + k < T ? row[i0 + k] : 0u;
-                sum += v[k];
+            for (int q = 0; q < kPer / 4; ++q) {
+                const uint4 x = reinterpret_cast<const uint4*>(row + i0)[q];
+                v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
             }
-            uint32_t agg;
-            uint32_t p = run + block_excl_scan<16>(sum, &agg, s_scan);
+        } else {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (i0 + k < T) row[i0 + k] = p;
-                p += v[k];
-            }
-            run += agg;
+            for (int k = 0; k < kPer; ++k) v[k] = 0;
         }
-        tot[half] = run;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if (i0 + k >= T) v[k] = 0;
+            sum += v[k];
+        }
+        const uint32_t inc = wave_incl_scan(sum);
+        if (lane_id() == 63) s_w[w] = inc;
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t x = s_w[8 * half + j];
+            pre += (8 * half + j < w) ? x : 0u;
+            tot += x;
+        }
+        __syncthreads();
+        uint32_t p = run + pre + inc - sum;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if (i0 + k < T) row[i0 + k] = p;
+            p += v[k];
+        }
+        run += tot;
     }
+    if (tt == 0) s_tot[half] = run;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        const uint64_t total = (uint64_t)tot[0] + tot[1];
-        sums[2 * (size_t)T] = tot[0];
-        sums[2 * (size_t)T + 1] = tot[1];
+        const uint32_t ta = s_tot[0], tr = s_tot[1];
+        const uint64_t total = (uint64_t)ta + tr;
+        sums[2 * (size_t)Tp] = ta;
+        sums[2 * (size_t)Tp + 1] = tr;
         h->need_P2 = (uint32_t)min(total, (uint64_t)0xFFFFFFFFu);
         if (total > a.cap_P2) {
             atomicOr(&h->status, ST_CAP_P2);
             h->P2 = a.cap_P2;
-            h->P2_valid = min(tot[0], a.cap_P2);
+            h->P2_valid = min(ta, a.cap_P2);
         } else {
             h->P2 = (uint32_t)total;
-            h->P2_valid = tot[0];
+            h->P2_valid = ta;
         }
     }
 }
@@ -737,37 +798,30 @@ SG_D void lds_sort_pass(const uint32_t* in, uint32_t* out, uint32_t n, int shift
     }
 }
 
-// Where tile b's pairs go: prefixes from k_leaf_scan, tile bounds, key base
-// and local key span; `staged` when the tile fits the LDS sort.
+// Where tile b's pairs go: prefixes from k_leaf_scan, tile bounds and leaf
+// key range from k_leaf_count; `staged` when the tile fits the LDS sort.
 struct LeafTilePlan {
     uint32_t a0, a1, pre_acc, pre_rej, tot_acc, kbase, span;
     bool staged;
 };
 
-SG_D bool leaf_tile_plan(const GridHeader* h, const BuildArgs& a, const LeafParams& L,
-                         const uint32_t* __restrict__ top_keys, const uint32_t* __restrict__ ranges,
-                         const uint32_t* __restrict__ cells, const uint32_t* __restrict__ sums, uint32_t T,
-                         uint32_t b, LeafTilePlan& q) {
-    const uint32_t tot_acc = sums[2 * (size_t)T], tot_rej = sums[2 * (size_t)T + 1];
+SG_D bool leaf_tile_plan(const BuildArgs& a, const uint32_t* __restrict__ sums, const uint32_t* __restrict__ info,
+                         uint32_t T, uint32_t b, LeafTilePlan& q) {
+    const uint32_t Tp = leaf_sums_stride(T);
+    const uint32_t tot_acc = sums[2 * (size_t)Tp], tot_rej = sums[2 * (size_t)Tp + 1];
     if ((uint64_t)tot_acc + tot_rej > a.cap_P2) return false;   // flagged by k_leaf_scan
+    const uint4 inf = reinterpret_cast<const uint4*>(info)[b];
+    if (inf.x == 0) return false;   // no top cell starts in this tile
+    q.a0 = ~inf.x;
+    q.a1 = inf.y > q.a0 ? inf.y : q.a0;
+    if (q.a0 >= q.a1) return false;
     q.tot_acc = tot_acc;
     q.pre_acc = sums[b];
-    q.pre_rej = sums[T + b];
+    q.pre_rej = sums[Tp + b];
     const uint32_t n_tile = ((b + 1 < T ? sums[b + 1] : tot_acc) - q.pre_acc) +
-                            ((b + 1 < T ? sums[T + b + 1] : tot_rej) - q.pre_rej);
-    leaf_tile(h, top_keys, ranges, b, &q.a0, &q.a1);
-    if (q.a0 >= q.a1) return false;
-    // tile-local keys: [0, span) for the leaf cells of the tile's top cells
-    const uint32_t k_first = top_keys[q.a0], k_last = top_keys[q.a1 - 1];
-    q.kbase = 0;
-    q.span = 0xFFFFFFFFu;
-    if (k_first < L.C1 && k_last < L.C1) {
-        const uint2 cf = reinterpret_cast<const uint2*>(cells)[k_first];
-        const uint2 cl = reinterpret_cast<const uint2*>(cells)[k_last];
-        const uint32_t nl = (cl.y & 0xFFu) * ((cl.y >> 8) & 0xFFu) * ((cl.y >> 16) & 0xFFu);
-        q.kbase = cf.x;
-        if (cl.x >= cf.x) q.span = cl.x - cf.x + nl;
-    }
+                            ((b + 1 < T ? sums[Tp + b + 1] : tot_rej) - q.pre_rej);
+    q.kbase = ~inf.z;
+    q.span = inf.w >= q.kbase ? inf.w - q.kbase : 0xFFFFFFFFu;
     q.staged = n_tile <= kLeafStage && q.span < (1u << kLocalBits) && q.a1 - q.a0 <= (1u << kPayBits);
     return true;
 }
@@ -776,17 +830,17 @@ SG_D bool leaf_tile_plan(const GridHeader* h, const BuildArgs& a, const LeafPara
 __global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* h, BuildArgs a,
                                                          const uint32_t* __restrict__ top_keys,
                                                          const uint32_t* __restrict__ top_vals,
-                                                         const uint32_t* __restrict__ ranges,
                                                          const uint32_t* __restrict__ cells,
                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                         const uint32_t* __restrict__ sums, uint32_t T) {
+                                                         const uint32_t* __restrict__ sums,
+                                                         const uint32_t* __restrict__ info, uint32_t T) {
     constexpr int NW = kLeafBlock / 64;
     __shared__ uint32_t s_scan[NW];
     __shared__ uint32_t s_a[kLeafStage], s_b[kLeafStage];
     __shared__ uint32_t s_base[1 << kDigitBits], s_wc[(1 << kDigitBits) * NW];
-    const LeafParams L = read_leaf_params(h);
     LeafTilePlan q;
-    if (!leaf_tile_plan(h, a, L, top_keys, ranges, cells, sums, T, blockIdx.x, q) || !q.staged) return;
+    if (!leaf_tile_plan(a, sums, info, T, blockIdx.x, q) || !q.staged) return;
+    const LeafParams L = read_leaf_params(h);
     const uint32_t a0 = q.a0, a1 = q.a1, kbase = q.kbase, span = q.span;
     const uint32_t cap = a.cap_P2;
     const uint32_t rej_base = q.tot_acc + q.pre_rej;
@@ -847,15 +901,15 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* 
 __global__ __launch_bounds__(kBlock) void k_leaf_emit_large(TriSoA t, GridHeader* h, BuildArgs a,
                                                             const uint32_t* __restrict__ top_keys,
                                                             const uint32_t* __restrict__ top_vals,
-                                                            const uint32_t* __restrict__ ranges,
                                                             const uint32_t* __restrict__ cells,
                                                             uint32_t* __restrict__ keys,
                                                             uint32_t* __restrict__ vals,
-                                                            const uint32_t* __restrict__ sums, uint32_t T) {
+                                                            const uint32_t* __restrict__ sums,
+                                                            const uint32_t* __restrict__ info, uint32_t T) {
     __shared__ uint32_t s_scan[kBlock / 64];
-    const LeafParams L = read_leaf_params(h);
     LeafTilePlan q;
-    if (!leaf_tile_plan(h, a, L, top_keys, ranges, cells, sums, T, blockIdx.x, q) || q.staged) return;
+    if (!leaf_tile_plan(a, sums, info, T, blockIdx.x, q) || q.staged) return;
+    const LeafParams L = read_leaf_params(h);
     const uint32_t cap = a.cap_P2;
     const uint32_t rej_base = q.tot_acc + q.pre_rej;
     if (threadIdx.x == 0) h->leaf_sort = q.tot_acc;
@@ -908,7 +962,7 @@ void launch_init(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, hipStream
 }
 
 size_t bounds_blocks(uint32_t n) {
-    const size_t b = ((size_t)n + kBlock * 4 - 1) / (kBlock * 4);
+    const size_t b = ((size_t)n + kBlock * 4 - 1) / (kBlock * 4);   // four 256-triangle chunks per block
     return b < 1 ? 1 : (b > 4096 ? 4096 : b);
 }
 
@@ -947,15 +1001,17 @@ void launch_leaf_cells(GridHeader* h, const BuildArgs& a, const uint32_t* ranges
 
 void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const uint32_t* top_keys,
                       const uint32_t* top_vals, const uint32_t* ranges, const uint32_t* cells, uint32_t* keys,
-                      uint32_t* vals, uint32_t* dbg, uint32_t* sums, hipStream_t st) {
+                      uint32_t* vals, uint32_t* dbg, uint32_t* sums, uint32_t* info, hipStream_t st) {
     const uint32_t T = (uint32_t)leaf_tiles(a.cap_P1);
-    hipLaunchKernelGGL(k_leaf_count, dim3(T), dim3(kLeafBlock), 0, st, t, h, top_keys, top_vals, ranges, cells, sums, T,
-                       dbg);
+    const uint32_t blocks = (a.cap_P1 + kTile - 1) / kTile;
+    if (blocks)
+        hipLaunchKernelGGL(k_leaf_count, dim3(blocks), dim3(kBlock), 0, st, t, h, top_keys, top_vals, ranges, cells,
+                           sums, info, T, dbg);
     hipLaunchKernelGGL(k_leaf_scan, dim3(1), dim3(1024), 0, st, h, a, sums, T);
-    hipLaunchKernelGGL(k_leaf_emit, dim3(T), dim3(kLeafBlock), 0, st, t, h, a, top_keys, top_vals, ranges, cells, keys,
-                       vals, sums, T);
-    hipLaunchKernelGGL(k_leaf_emit_large, dim3(T), dim3(kBlock), 0, st, t, h, a, top_keys, top_vals, ranges, cells,
-                       keys, vals, sums, T);
+    hipLaunchKernelGGL(k_leaf_emit, dim3(T), dim3(kLeafBlock), 0, st, t, h, a, top_keys, top_vals, cells, keys, vals,
+                       sums, info, T);
+    hipLaunchKernelGGL(k_leaf_emit_large, dim3(T), dim3(kBlock), 0, st, t, h, a, top_keys, top_vals, cells, keys,
+                       vals, sums, info, T);
 }
 
 void launch_leaf_sort_fixup(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, GridHeader* h, uint32_t cap,
@@ -971,11 +1027,13 @@ void launch_leaf_sort_fixup(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* 
 // number of per-tile sums the emission kernels need for n items
 size_t emit_tiles(uint32_t n) { return ((size_t)n + kTile - 1) / kTile; }
 size_t cell_tiles(uint32_t n) { return ((size_t)n + kBlock - 1) / kBlock; }
-// leaf tiles over n sorted top pairs (the leaf sums hold 2 * tiles + 2 words)
+// leaf tiles over n sorted top pairs
 size_t leaf_tiles(uint32_t n) {
     const size_t t = ((size_t)n + kLeafTile - 1) / kLeafTile;
     return t ? t : 1;
 }
+size_t leaf_sums_words(uint32_t n) { return 2 * (size_t)leaf_sums_stride((uint32_t)leaf_tiles(n)) + 4; }
+size_t leaf_info_words(uint32_t n) { return 4 * leaf_tiles(n); }
 
 
 }  // namespace sg

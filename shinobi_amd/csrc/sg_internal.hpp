@@ -43,11 +43,15 @@ struct sg_grid {
     uint32_t* leaves = nullptr;
     uint32_t *dbg_tri = nullptr, *dbg_leafc = nullptr, *dbg_pair = nullptr;
     uint32_t *sums_top = nullptr, *sums_cells = nullptr, *sums_leaf = nullptr;   // per-tile sums
+    uint32_t* leaf_info = nullptr;                                               // per leaf tile bounds
     uint32_t* bpart = nullptr;                                                   // bounds partials
     uint32_t *sort1 = nullptr, *sort2 = nullptr;
     size_t sort1_words = 0, sort2_words = 0;
     int key_bits1 = 1, key_bits2 = 1, passes1 = 1, passes2 = 1, rb = 8;
     hipStream_t stream = nullptr;
+    // queues of the compacting multi-pass traversal (grown on demand)
+    void* tscratch = nullptr;
+    size_t tscratch_bytes = 0;
 };
 
 #define HIPCHK(x)                                   \

@@ -77,11 +77,15 @@ void launch_ranges(const uint32_t* keys, const uint32_t* d_n, uint32_t cap, cons
                    uint32_t* cells, uint32_t* d_pvalid, hipStream_t st);
 void launch_leaf_cells(GridHeader* h, const BuildArgs& a, const uint32_t* ranges, uint32_t* cells,
                        uint32_t* dbg, uint32_t* sums, hipStream_t st);
-// leaf pairs, sorted by key within each leaf tile; sums: 2 * leaf_tiles(cap_P1) + 2 words
+// leaf pairs, sorted by key within each leaf tile.  sums
+// (leaf_sums_words(cap_P1)) and info (leaf_info_words(cap_P1)) must be zero
+// on entry (k_init zeroes them).
 size_t leaf_tiles(uint32_t n);
+size_t leaf_sums_words(uint32_t n);
+size_t leaf_info_words(uint32_t n);
 void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const uint32_t* top_keys,
                       const uint32_t* top_vals, const uint32_t* ranges, const uint32_t* cells, uint32_t* keys,
-                      uint32_t* vals, uint32_t* dbg, uint32_t* sums, hipStream_t st);
+                      uint32_t* vals, uint32_t* dbg, uint32_t* sums, uint32_t* info, hipStream_t st);
 // global stable sort of the leaf pairs of tiles too large for LDS (no work
 // when there are none); the result stays in (k0, v0)
 void launch_leaf_sort_fixup(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, GridHeader* h, uint32_t cap,
@@ -120,10 +124,14 @@ struct TraceItem {
 // launches can be captured in a graph and replayed
 void launch_trace_items(const TraceItem* items, uint32_t n_items, uint32_t total_chunks, uint32_t* counter,
                         bool shadow, hipStream_t st);
+void init_trace_items();
 // closest hit, or with shadow the any-hit traversal that stops at t < 0.9999
-// (block: threads per block, 64, 128 or 256)
+// (block: threads per block, 64, 128 or 256).  With scratch
+// (trace_scratch_bytes(n) bytes) and no stats the walk runs in compacting
+// passes; without, in one pass.  Results are the same.
+size_t trace_scratch_bytes(uint32_t n);
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim,
-                  float* t, uint32_t* stats, bool shadow, int block, hipStream_t st);
+                  float* t, uint32_t* stats, bool shadow, int block, void* scratch, hipStream_t st);
 // one cosine-weighted bounce per hit, compacted in primary order; *d_count
 // receives the number of bounce rays.  scratch: bounce_scratch_bytes(n).
 size_t bounce_scratch_bytes(uint32_t n);

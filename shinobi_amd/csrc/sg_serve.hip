@@ -53,6 +53,7 @@ sg_status sg_batch_create(sg_ctx* ctx, uint32_t max_items, sg_batch** out) {
     b->ctx = ctx;
     b->cap = max_items;
     b->items.reserve(max_items);
+    init_trace_items();
     if (hipHostMalloc((void**)&b->pinned, sizeof(TraceItem) * max_items, hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void**)&b->dev, sizeof(TraceItem) * max_items) != hipSuccess ||
         hipMalloc((void**)&b->counter, 256) != hipSuccess || hipEventCreate(&b->uploaded) != hipSuccess) {
@@ -368,7 +369,6 @@ struct sg_server {
     sg_ctx* ctx = nullptr;
     sg_server_config cfg{};
     char* dev = nullptr;       // device arena (hbm_budget bytes)
-    char* host = nullptr;      // pinned mirror of the arena
     Arena arena;
     std::vector<Slot> slots;
     std::vector<SceneEntry> scenes;
@@ -392,37 +392,40 @@ struct sg_server {
 
 namespace {
 
-// copy the wave's requests into the pinned mirror, upload, trace, download
+// upload the wave's rays straight from the callers' buffers (one copy when
+// a request's seven arrays are one contiguous SoA block), trace, and copy
+// the results straight back; the copies are DMA transfers when the callers'
+// memory is pinned (sg_host_alloc)
 sg_status launch_wave(sg_server* s, Slot& sl) {
     const bool plain = s->cfg.mode == SG_SERVE_PLAIN;
-    char* hbase = s->host + sl.off;
     char* dbase = s->dev + sl.off;
-    // layout: [rays of every request][prim, t of every request]
+    // device layout: [rays of every request][prim, t of every request]
     size_t ro = 0;
     std::vector<size_t> rofs(sl.reqs.size());
     for (size_t k = 0; k < sl.reqs.size(); ++k) {
         rofs[k] = ro;
         ro += align_up((size_t)sl.reqs[k]->n * 4, kAlign) * 7;
     }
-    sl.rays_bytes = ro;
     size_t oo = ro;
     std::vector<size_t> oofs(sl.reqs.size());
     for (size_t k = 0; k < sl.reqs.size(); ++k) {
         oofs[k] = oo;
         oo += align_up((size_t)sl.reqs[k]->n * 4, kAlign) * 2;
     }
-    for (size_t k = 0; k < sl.reqs.size(); ++k) {
-        const Request* r = sl.reqs[k];
-        const size_t a = align_up((size_t)r->n * 4, kAlign);
-        for (int c = 0; c < 7; ++c) memcpy(hbase + rofs[k] + c * a, r->rays[c], (size_t)r->n * 4);
-    }
-    HIPCHK(hipMemcpyAsync(dbase, hbase, ro, hipMemcpyHostToDevice, sl.st));
     uint32_t chunk = 0;
     for (size_t k = 0; k < sl.reqs.size(); ++k) {
         const Request* r = sl.reqs[k];
-        SceneEntry& se = s->scenes[r->scene];
+        const size_t nb = (size_t)r->n * 4, a = align_up(nb, kAlign);
+        bool contiguous = a == nb;
+        for (int c = 1; c < 7 && contiguous; ++c) contiguous = r->rays[c] == r->rays[0] + (size_t)c * r->n;
+        if (contiguous) {
+            HIPCHK(hipMemcpyAsync(dbase + rofs[k], r->rays[0], 7 * nb, hipMemcpyHostToDevice, sl.st));
+        } else {
+            for (int c = 0; c < 7; ++c)
+                HIPCHK(hipMemcpyAsync(dbase + rofs[k] + c * a, r->rays[c], nb, hipMemcpyHostToDevice, sl.st));
+        }
+        const SceneEntry& se = s->scenes[r->scene];
         const int gi = r->gi;
-        const size_t a = align_up((size_t)r->n * 4, kAlign);
         TraceItem it;
         it.g = grid_view(se.grid[gi]);
         for (int c = 0; c < 7; ++c) it.rays.a[c] = reinterpret_cast<const float*>(dbase + rofs[k] + c * a);
@@ -433,7 +436,7 @@ sg_status launch_wave(sg_server* s, Slot& sl) {
         chunk += chunks_of(r->n);
         if (plain) {
             launch_trace(it.g, it.rays, it.n, nullptr, it.prim, it.t, nullptr, false, se.grid[gi]->p.trace_block,
-                         sl.st);
+                         nullptr, sl.st);
         } else {
             sl.pinned_items[k] = it;
         }
@@ -444,21 +447,14 @@ sg_status launch_wave(sg_server* s, Slot& sl) {
         launch_trace_items(sl.dev_items, (uint32_t)sl.reqs.size(), chunk, sl.counter, false, sl.st);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(hbase + ro, dbase + ro, oo - ro, hipMemcpyDeviceToHost, sl.st));
+    for (size_t k = 0; k < sl.reqs.size(); ++k) {
+        const Request* r = sl.reqs[k];
+        const size_t nb = (size_t)r->n * 4, a = align_up(nb, kAlign);
+        HIPCHK(hipMemcpyAsync(r->prim_out, dbase + oofs[k], nb, hipMemcpyDeviceToHost, sl.st));
+        HIPCHK(hipMemcpyAsync(r->t_out, dbase + oofs[k] + a, nb, hipMemcpyDeviceToHost, sl.st));
+    }
     HIPCHK(hipEventRecord(sl.ev, sl.st));
     return SG_OK;
-}
-
-// results to the callers' buffers; called without the lock held
-void finish_wave(sg_server* s, Slot& sl) {
-    const char* hbase = s->host + sl.off;
-    size_t oo = sl.rays_bytes;
-    for (Request* r : sl.reqs) {
-        const size_t a = align_up((size_t)r->n * 4, kAlign);
-        memcpy(r->prim_out, hbase + oo, (size_t)r->n * 4);
-        memcpy(r->t_out, hbase + oo + a, (size_t)r->n * 4);
-        oo += 2 * a;
-    }
 }
 
 void serve_loop(sg_server* s) {
@@ -470,10 +466,7 @@ void serve_loop(sg_server* s) {
         for (Slot& sl : s->slots) {
             if (!sl.busy) continue;
             if (hipEventQuery(sl.ev) != hipSuccess) continue;
-            lk.unlock();
-            finish_wave(s, sl);
             const auto now = Clock::now();
-            lk.lock();
             for (Request* r : sl.reqs) {
                 s->scenes[r->scene].in_use[r->gi]--;
                 r->t_done = now;
@@ -577,7 +570,6 @@ sg_status sg_server_create(sg_ctx* ctx, const sg_server_config* cfg, sg_server**
     s->cfg.hbm_budget = budget;
     s->arena = Arena(budget);
     bool ok = hipMalloc((void**)&s->dev, budget) == hipSuccess &&
-              hipHostMalloc((void**)&s->host, budget, hipHostMallocDefault) == hipSuccess &&
               hipStreamCreateWithFlags(&s->rebuild_st, hipStreamNonBlocking) == hipSuccess;
     s->slots.resize(s->cfg.max_waves);
     for (Slot& sl : s->slots) {
@@ -593,6 +585,7 @@ sg_status sg_server_create(sg_ctx* ctx, const sg_server_config* cfg, sg_server**
         sg_server_destroy(s);
         return SG_ERR_OUT_OF_MEMORY;
     }
+    init_trace_items();
     s->worker = std::thread(serve_loop, s);
     *out = s;
     return SG_OK;
@@ -628,7 +621,6 @@ sg_status sg_server_destroy(sg_server* s) {
     for (Request* r : s->pending) delete r;
     if (s->rebuild_st) hipStreamDestroy(s->rebuild_st);
     if (s->dev) hipFree(s->dev);
-    if (s->host) hipHostFree(s->host);
     delete s;
     return SG_OK;
 }

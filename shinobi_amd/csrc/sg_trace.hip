@@ -123,37 +123,53 @@ SG_D bool traverse_init(const GParams& p, const Ray& r, float& tEntry, v3& cell,
 // SimpleGridTraverser.h:436)
 constexpr float kShadowT = 0.9999f;
 
+// The grid walk of one ray, resumable at top-level (uniform: cell) steps:
+// cell, tMax, tEntry and the step guard are the whole state between steps.
+struct Walk {
+    v3 cell, tMax;
+    float tEntry;
+    uint32_t guard;
+    bool active;
+};
+
+// a ray crosses at most res.x+res.y+res.z cells of a level; the cap only
+// ends walks that can never reach the exit test (NaN / inf rays, entry cell
+// past the far boundary), which the reference would loop on forever
+SG_D uint32_t step_cap(const GParams& p) { return 2u * (uint32_t)(p.res[0] + p.res[1] + p.res[2]) + 16u; }
+
 // UniformGridTraverser::traverse, src/Control/Algorithm/SimpleGridTraverser.h:274-401;
 // SHADOW: UniformGridShadowTraverser (:409-464) starts only if t >= 0.9999,
 // stops once t < 0.9999 and otherwise walks to the grid exit
 template <bool SHADOW>
-SG_D void trace_uniform(const GridView& g, const GParams& p, Ray& r, uint32_t& steps) {
-    float tEntry;
-    v3 cell, tMax;
-    bool flag = (SHADOW ? r.t >= kShadowT : r.t > 0.f) && traverse_init<false>(p, r, tEntry, cell, tMax);
-    // a ray crosses at most res.x+res.y+res.z cells; the cap only ends walks
-    // that can never reach the exit test (NaN / inf rays, entry cell past
-    // the far boundary), which the reference would loop on forever
-    const uint32_t cap = 2u * (uint32_t)(p.res[0] + p.res[1] + p.res[2]) + 16u;
-    uint32_t guard = 0;
-    while (flag && guard++ < cap) {
-        const uint32_t k = clamp_idx(cell.x, p.res[0]) + clamp_idx(cell.y, p.res[1]) * (uint32_t)p.res[0] +
-                           clamp_idx(cell.z, p.res[2]) * (uint32_t)p.res[0] * (uint32_t)p.res[1];
+SG_D void walk_init_uniform(const GParams& p, const Ray& r, Walk& w) {
+    w.guard = 0;
+    w.active = (SHADOW ? r.t >= kShadowT : r.t > 0.f) && traverse_init<false>(p, r, w.tEntry, w.cell, w.tMax);
+}
+
+// at most `budget` cells of the walk
+template <bool SHADOW>
+SG_D void walk_uniform(const GridView& g, const GParams& p, Ray& r, Walk& w, uint32_t budget, uint32_t& steps) {
+    const uint32_t cap = step_cap(p);
+    for (uint32_t it = 0; it < budget && w.active; ++it) {
+        if (!(w.guard++ < cap)) {
+            w.active = false;
+            break;
+        }
+        const uint32_t k = clamp_idx(w.cell.x, p.res[0]) + clamp_idx(w.cell.y, p.res[1]) * (uint32_t)p.res[0] +
+                           clamp_idx(w.cell.z, p.res[2]) * (uint32_t)p.res[0] * (uint32_t)p.res[1];
         const uint2 rg = *reinterpret_cast<const uint2*>(g.cells + 2 * (size_t)k);
         intersect(g.indices, g.rec, rg.x, rg.y, r);
-        if (SHADOW) {
-            if (!(r.t >= kShadowT)) break;
-        } else {
-            flag = r.t > tMax.x || r.t > tMax.y || r.t > tMax.z;
-            if (!flag) break;
+        if (SHADOW ? !(r.t >= kShadowT) : !(r.t > w.tMax.x || r.t > w.tMax.y || r.t > w.tMax.z)) {
+            w.active = false;
+            break;
         }
         ++steps;
-        const int d = min_dim(tMax);
+        const int d = min_dim(w.tMax);
         const float rc = comp(r.rcp, d);
-        setc(cell, d, comp(cell, d) + (rc > 0.f ? 1.f : -1.f));
-        setc(tMax, d, comp(tMax, d) + comp(p.cs, d) * fabsf(rc));
-        const float cd = comp(cell, d);
-        flag = !(fabsf(cd - (float)p.res[d]) < 0.1f || fabsf(cd + 1.f) < 0.1f);
+        setc(w.cell, d, comp(w.cell, d) + (rc > 0.f ? 1.f : -1.f));
+        setc(w.tMax, d, comp(w.tMax, d) + comp(p.cs, d) * fabsf(rc));
+        const float cd = comp(w.cell, d);
+        w.active = !(fabsf(cd - (float)p.res[d]) < 0.1f || fabsf(cd + 1.f) < 0.1f);
     }
 }
 
@@ -164,20 +180,30 @@ SG_D void trace_uniform(const GridView& g, const GParams& p, Ray& r, uint32_t& s
 // a failed traverseInit and walks from an uninitialised cell; here such a
 // ray is not traversed.
 template <bool SHADOW>
-SG_D void trace_twolevel(const GridView& g, const GParams& p, Ray& r, uint32_t& lsteps, uint32_t& tsteps,
-                         uint32_t& entered) {
-    float tEntry;
-    v3 cell, tMax;
-    bool flag1 = SHADOW ? !(r.t < kShadowT) : r.t > 0.f;
-    if (!traverse_init<true>(p, r, tEntry, cell, tMax)) flag1 = false;
-    const uint32_t cap = 2u * (uint32_t)(p.res[0] + p.res[1] + p.res[2]) + 16u;
-    uint32_t guard = 0;
-    while (flag1 && guard++ < cap) {
+SG_D void walk_init_twolevel(const GParams& p, const Ray& r, Walk& w) {
+    w.guard = 0;
+    w.active = SHADOW ? !(r.t < kShadowT) : r.t > 0.f;
+    if (!traverse_init<true>(p, r, w.tEntry, w.cell, w.tMax)) w.active = false;
+}
+
+// at most `budget` top cells of the walk (each entered top cell's leaf walk
+// runs to its end)
+template <bool SHADOW>
+SG_D void walk_twolevel(const GridView& g, const GParams& p, Ray& r, Walk& w, uint32_t budget, uint32_t& lsteps,
+                        uint32_t& tsteps, uint32_t& entered) {
+    const uint32_t cap = step_cap(p);
+    for (uint32_t it = 0; it < budget && w.active; ++it) {
+        if (!(w.guard++ < cap)) {
+            w.active = false;
+            break;
+        }
+        const v3 cell = w.cell;
         const uint32_t k = clamp_idx(cell.x, p.res[0]) + clamp_idx(cell.y, p.res[1]) * (uint32_t)p.res[0] +
                            clamp_idx(cell.z, p.res[2]) * (uint32_t)p.res[0] * (uint32_t)p.res[1];
         const uint2 top = *reinterpret_cast<const uint2*>(g.cells + 2 * (size_t)k);
         if (top.y & kNotEmpty) {
             ++entered;
+            const float tEntry = w.tEntry;
             const v3 ep = r.o + mk(tEntry, tEntry, tEntry) / r.rcp;
             const uint32_t c0 = top.y & 0xFFu, c1 = (top.y >> 8) & 0xFFu, c2 = (top.y >> 16) & 0xFFu;
             const v3 cres = mk((float)c0, (float)c1, (float)c2);
@@ -214,20 +240,18 @@ SG_D void trace_twolevel(const GridView& g, const GParams& p, Ray& r, uint32_t& 
                 flag2 = !(fabsf(sd - comp(cres, d)) < 0.1f || fabsf(sd + 1.f) < 0.1f);
             }
         }
-        if (SHADOW) {
-            if (r.t < kShadowT) break;
-        } else {
-            flag1 = r.t > tMax.x || r.t > tMax.y || r.t > tMax.z;
-            if (!flag1) break;
+        if (SHADOW ? r.t < kShadowT : !(r.t > w.tMax.x || r.t > w.tMax.y || r.t > w.tMax.z)) {
+            w.active = false;
+            break;
         }
         ++tsteps;
-        const int d = min_dim(tMax);
+        const int d = min_dim(w.tMax);
         const float rc = comp(r.rcp, d);
-        tEntry = comp(tMax, d);
-        setc(cell, d, comp(cell, d) + (rc > 0.f ? 1.f : -1.f));
-        setc(tMax, d, comp(tMax, d) + comp(p.cs, d) * fabsf(rc));
-        const float cd = comp(cell, d);
-        flag1 = !(fabsf(cd - (float)p.res[d]) < 0.1f || fabsf(cd + 1.f) < 0.1f);
+        w.tEntry = comp(w.tMax, d);
+        setc(w.cell, d, comp(w.cell, d) + (rc > 0.f ? 1.f : -1.f));
+        setc(w.tMax, d, comp(w.tMax, d) + comp(p.cs, d) * fabsf(rc));
+        const float cd = comp(w.cell, d);
+        w.active = !(fabsf(cd - (float)p.res[d]) < 0.1f || fabsf(cd + 1.f) < 0.1f);
     }
 }
 
@@ -254,8 +278,14 @@ SG_D void trace_one(const GridView& g, const RaySoA& rays, uint32_t i, uint32_t*
     r.best = kMiss;
     r.tests = 0;
     uint32_t s1 = 0, s2 = 0, s3 = 0;
-    if (KIND == 0) trace_uniform<SHADOW>(g, p, r, s1);
-    else trace_twolevel<SHADOW>(g, p, r, s1, s2, s3);
+    Walk w;
+    if (KIND == 0) {
+        walk_init_uniform<SHADOW>(p, r, w);
+        walk_uniform<SHADOW>(g, p, r, w, 0xFFFFFFFFu, s1);
+    } else {
+        walk_init_twolevel<SHADOW>(p, r, w);
+        walk_twolevel<SHADOW>(g, p, r, w, 0xFFFFFFFFu, s1, s2, s3);
+    }
     prim[i] = r.best == kMiss ? kMiss : g.indices[r.best];
     tout[i] = r.t;
     if (STATS) {
@@ -277,22 +307,161 @@ __global__ __launch_bounds__(kBlock) void k_trace(GridView g, RaySoA rays, uint3
     trace_one<KIND, STATS, SHADOW>(g, rays, i, prim, tout, stats);
 }
 
+// ------------------------------------------------- compacting multi-pass walk
+// One ray per lane keeps neighbouring rays in one wave, but a wave runs as
+// long as its longest ray: on incoherent (bounce) rays the longest of 64
+// walks is several times the mean.  The walk therefore runs in passes: each
+// pass advances every live ray by at most `budget` top-level (uniform: cell)
+// steps, rays that finish store their result, and the others are appended,
+// wave by wave, to a compacted queue with their walk state, which the next
+// pass resumes with full waves.  The last pass has no budget.  Every ray
+// takes exactly the steps of the one-pass walk, so results are identical.
+struct WalkSave {
+    float t;
+    uint32_t best;
+    float cell[3], tmax[3];
+    float tEntry;
+    uint32_t guard;
+};
+
+template <int KIND, bool SHADOW>
+__global__ __launch_bounds__(kBlock) void k_trace_pass(GridView g, RaySoA rays, uint32_t cap, const uint32_t* d_n,
+                                                       uint32_t* __restrict__ prim, float* __restrict__ tout,
+                                                       const uint32_t* __restrict__ qin,
+                                                       const WalkSave* __restrict__ sin, const uint32_t* qin_n,
+                                                       uint32_t* __restrict__ qout, WalkSave* __restrict__ sout,
+                                                       uint32_t* qout_n, uint32_t budget) {
+    const uint32_t n = qin ? min(*qin_n, cap) : (d_n ? min(*d_n, cap) : cap);
+    if (blockIdx.x * blockDim.x >= n) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool more = false;
+    uint32_t id = 0;
+    Ray r;
+    Walk w;
+    if (i < n) {
+        id = qin ? qin[i] : i;
+        if (g.hdr->status & ~(ST_LEAF_RES | ST_NEG_EXTENT)) {
+            // a failed or overflowed build is never traversed: every ray misses
+            prim[id] = kMiss;
+            tout[id] = rays.a[6][id];
+        } else {
+            const GParams p = load_gparams(g.hdr);
+            r.o = mk(rays.a[0][id], rays.a[1][id], rays.a[2][id]);
+            const v3 d = mk(rays.a[3][id], rays.a[4][id], rays.a[5][id]);
+            r.rcp = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+            r.D = mk(1.f / r.rcp.x, 1.f / r.rcp.y, 1.f / r.rcp.z);
+            r.tests = 0;
+            if (qin) {
+                const WalkSave sv = sin[i];
+                r.t = sv.t;
+                r.best = sv.best;
+                w.cell = mk(sv.cell[0], sv.cell[1], sv.cell[2]);
+                w.tMax = mk(sv.tmax[0], sv.tmax[1], sv.tmax[2]);
+                w.tEntry = sv.tEntry;
+                w.guard = sv.guard;
+                w.active = true;
+            } else {
+                r.t = rays.a[6][id];
+                r.best = kMiss;
+                if (KIND == 0) walk_init_uniform<SHADOW>(p, r, w);
+                else walk_init_twolevel<SHADOW>(p, r, w);
+            }
+            uint32_t s1 = 0, s2 = 0, s3 = 0;
+            if (KIND == 0) walk_uniform<SHADOW>(g, p, r, w, budget, s1);
+            else walk_twolevel<SHADOW>(g, p, r, w, budget, s1, s2, s3);
+            if (w.active && qout) {
+                more = true;
+            } else {
+                prim[id] = r.best == kMiss ? kMiss : g.indices[r.best];
+                tout[id] = r.t;
+            }
+        }
+    }
+    const uint64_t m = __ballot(more);
+    if (m) {
+        const int leader = __ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if (lane_id() == leader) base = atomicAdd(qout_n, (uint32_t)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if (more) {
+            const uint32_t pos = base + (uint32_t)__popcll(m & lanemask_lt());
+            qout[pos] = id;
+            WalkSave sv;
+            sv.t = r.t;
+            sv.best = r.best;
+            sv.cell[0] = w.cell.x;
+            sv.cell[1] = w.cell.y;
+            sv.cell[2] = w.cell.z;
+            sv.tmax[0] = w.tMax.x;
+            sv.tmax[1] = w.tMax.y;
+            sv.tmax[2] = w.tMax.z;
+            sv.tEntry = w.tEntry;
+            sv.guard = w.guard;
+            sout[pos] = sv;
+        }
+    }
+}
+
+__global__ void k_set_counter(uint32_t* p) {
+    if (threadIdx.x == 0) *p = 0u;
+}
+
+// step budgets of the passes (0: unlimited); top-level cells for the
+// two-level grid, cells for the uniform grid
+constexpr uint32_t kPassBudget[2][4] = {{32, 32, 64, 0}, {8, 8, 16, 0}};
+constexpr int kPasses = 4;
+
+size_t trace_scratch_bytes(uint32_t n) {
+    const size_t ib = ((size_t)n * sizeof(uint32_t) + 255) & ~(size_t)255;
+    const size_t sb = ((size_t)n * sizeof(WalkSave) + 255) & ~(size_t)255;
+    return 256 + 2 * ib + 2 * sb;
+}
+
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim, float* t,
-                  uint32_t* stats, bool shadow, int block, hipStream_t st) {
+                  uint32_t* stats, bool shadow, int block, void* scratch, hipStream_t st) {
     if (n == 0) return;
     // one ray per lane keeps the 64 lanes of a wave on neighbouring rays, so
     // coherent rays share cell, leaf and triangle loads
     const int B = (block == 64 || block == 128 || block == 256) ? block : kBlock;
     const dim3 grid((n + B - 1) / B), tb(B);
+    if (stats || !scratch) {
 #define SG_TRACE(K, S, H) hipLaunchKernelGGL((k_trace<K, S, H>), grid, tb, 0, st, g, r, n, d_n, prim, t, stats)
-    if (g.kind == 0) {
-        if (shadow) { if (stats) SG_TRACE(0, true, true); else SG_TRACE(0, false, true); }
-        else { if (stats) SG_TRACE(0, true, false); else SG_TRACE(0, false, false); }
-    } else {
-        if (shadow) { if (stats) SG_TRACE(1, true, true); else SG_TRACE(1, false, true); }
-        else { if (stats) SG_TRACE(1, true, false); else SG_TRACE(1, false, false); }
-    }
+        if (g.kind == 0) {
+            if (shadow) { if (stats) SG_TRACE(0, true, true); else SG_TRACE(0, false, true); }
+            else { if (stats) SG_TRACE(0, true, false); else SG_TRACE(0, false, false); }
+        } else {
+            if (shadow) { if (stats) SG_TRACE(1, true, true); else SG_TRACE(1, false, true); }
+            else { if (stats) SG_TRACE(1, true, false); else SG_TRACE(1, false, false); }
+        }
 #undef SG_TRACE
+        return;
+    }
+    // scratch: [2 counters][ids A][ids B][states A][states B]
+    char* sp = static_cast<char*>(scratch);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(sp);
+    const size_t ib = ((size_t)n * sizeof(uint32_t) + 255) & ~(size_t)255;
+    const size_t sb = ((size_t)n * sizeof(WalkSave) + 255) & ~(size_t)255;
+    uint32_t* ids[2] = {reinterpret_cast<uint32_t*>(sp + 256), reinterpret_cast<uint32_t*>(sp + 256 + ib)};
+    WalkSave* sv[2] = {reinterpret_cast<WalkSave*>(sp + 256 + 2 * ib),
+                       reinterpret_cast<WalkSave*>(sp + 256 + 2 * ib + sb)};
+    const int kind = g.kind == 0 ? 0 : 1;
+    for (int pass = 0; pass < kPasses; ++pass) {
+        const uint32_t budget = kPassBudget[kind][pass] ? kPassBudget[kind][pass] : 0xFFFFFFFFu;
+        const bool last = pass == kPasses - 1;
+        const int in = (pass + 1) & 1, out = pass & 1;
+        const uint32_t* qin = pass ? ids[in] : nullptr;
+        const WalkSave* sin = pass ? sv[in] : nullptr;
+        const uint32_t* qin_n = pass ? cnt + in : nullptr;
+        uint32_t* qout = last ? nullptr : ids[out];
+        WalkSave* sout = last ? nullptr : sv[out];
+        uint32_t* qout_n = last ? nullptr : cnt + out;
+        if (qout_n) hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, st, qout_n);
+#define SG_PASS(K, H) hipLaunchKernelGGL((k_trace_pass<K, H>), grid, tb, 0, st, g, r, n, d_n, prim, t, qin, sin, \
+                                         qin_n, qout, sout, qout_n, budget)
+        if (kind == 0) { if (shadow) SG_PASS(0, true); else SG_PASS(0, false); }
+        else { if (shadow) SG_PASS(1, true); else SG_PASS(1, false); }
+#undef SG_PASS
+    }
 }
 
 // ------------------------------------------------- many batches, one launch
@@ -331,24 +500,25 @@ __global__ __launch_bounds__(kBlock) void k_trace_items(const TraceItem* __restr
     }
 }
 
-__global__ void k_set_counter(uint32_t* p) {
-    if (threadIdx.x == 0) *p = 0u;
-}
-
 static int g_item_blocks = 0;
+
+// resident blocks of k_trace_items on the current device (queried once,
+// outside any stream capture)
+void init_trace_items() {
+    if (g_item_blocks != 0) return;
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_trace_items), kBlock,
+                                                        0);
+    g_item_blocks = std::max(1, cus) * std::max(1, per);
+}
 
 void launch_trace_items(const TraceItem* items, uint32_t n_items, uint32_t total_chunks, uint32_t* counter,
                         bool shadow, hipStream_t st) {
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, st, counter);
     if (n_items == 0 || total_chunks == 0) return;
-    if (g_item_blocks == 0) {
-        int dev = 0, cus = 0, per = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_trace_items),
-                                                            kBlock, 0);
-        g_item_blocks = std::max(1, cus) * std::max(1, per);
-    }
+    init_trace_items();
     const uint32_t waves_needed = total_chunks;
     uint32_t blocks = (uint32_t)g_item_blocks;
     const uint32_t max_blocks = (waves_needed + kBlock / 64 - 1) / (kBlock / 64);
