@@ -180,11 +180,13 @@ sg_status sg_trace_shadow(sg_ctx *ctx, sg_grid *grid, const sg_rays *rays, uint3
                           const uint32_t *d_count, uint32_t *d_prim, float *d_t, uint32_t *d_stats,
                           void *stream);
 /* one cosine-weighted diffuse bounce per hit (SimplePathTracer.hpp:115-166),
- * compacted in primary order; *d_count = number of bounce rays; d_src (opt.)
- * = primary ray index of each bounce ray */
+ * compacted in input order; *d_count = number of bounce rays; d_src (opt.)
+ * = input ray index of each bounce ray.  n = capacity of the input; d_n
+ * (optional device pointer) = the number of input rays (<= n), so bounces
+ * chain path segments without reading counts back to the host. */
 sg_status sg_bounce(sg_ctx *ctx, sg_scene *scene, const sg_rays *rays, const uint32_t *d_prim,
-                    const float *d_t, uint32_t n, uint64_t seed, const sg_rays_out *out, uint32_t *d_src,
-                    uint32_t *d_count, void *stream);
+                    const float *d_t, uint32_t n, const uint32_t *d_n, uint64_t seed, const sg_rays_out *out,
+                    uint32_t *d_src, uint32_t *d_count, void *stream);
 
 /* ---------------------------------------------------------- primitives --- */
 /* ExclusiveScan / InclusiveScan (src/Utils/Scan.h:35-68) on device arrays */

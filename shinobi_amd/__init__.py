@@ -412,15 +412,43 @@ class FastTwoLevelGridBuilder(_Builder):
 
 
 def bounce(ctx: Context, scene: FaceSoup, rays: RayBatch, prim: DeviceBuffer, t: DeviceBuffer, seed: int,
-           out: RayBatch, src: DeviceBuffer | None = None, count: DeviceBuffer | None = None) -> DeviceBuffer:
+           out: RayBatch, src: DeviceBuffer | None = None, count: DeviceBuffer | None = None,
+           d_n: C.c_void_p | None = None) -> DeviceBuffer:
     """One cosine-weighted diffuse bounce per hit, compacted; returns the
-    device counter holding the number of bounce rays."""
+    device counter holding the number of bounce rays.  d_n: optional device
+    count of the input rays (<= rays.n)."""
     count = count or DeviceBuffer(ctx, 4)
     r = rays.struct()
     o = out.struct()
-    check(ctx._l.sg_bounce(ctx.h, scene.h, C.byref(r), prim.ptr, t.ptr, rays.n, seed, C.byref(o),
+    check(ctx._l.sg_bounce(ctx.h, scene.h, C.byref(r), prim.ptr, t.ptr, rays.n, d_n, seed, C.byref(o),
                            src.ptr if src is not None else None, count.ptr, ctx.stream), "sg_bounce")
     return count
+
+
+def trace_paths(ctx: Context, scene: FaceSoup, grid: "Grid", rays: RayBatch, depth: int, seed: int,
+                keep: bool = False):
+    """Diffuse paths: trace the rays, bounce every hit, trace the bounce rays,
+    ... `depth` traced segments in all, every count kept on the device
+    (sg_bounce with a device input count).  Segment d > 0 uses bounce seed
+    seed + d - 1.  Returns the device counters of segments 1..depth-1 (and,
+    with keep, each segment's (rays, prim, t))."""
+    n = rays.n
+    bufs = [RayBatch(ctx, capacity=n), RayBatch(ctx, capacity=n)]
+    prims = [DeviceBuffer(ctx, 4 * n) for _ in range(2)]
+    ts = [DeviceBuffer(ctx, 4 * n) for _ in range(2)]
+    counts = [DeviceBuffer(ctx, 4) for _ in range(max(depth - 1, 0))]
+    segs = []
+    cur, cur_n, k = rays, None, 0
+    for d in range(depth):
+        p, t = (prims[k], ts[k]) if not keep else (DeviceBuffer(ctx, 4 * n), DeviceBuffer(ctx, 4 * n))
+        grid.trace(cur, prim=p, t=t, d_count=cur_n, n=n)
+        if keep:
+            segs.append((cur, p, t, cur_n))
+        if d + 1 < depth:
+            nxt = bufs[k] if not keep else RayBatch(ctx, capacity=n)
+            bounce(ctx, scene, cur, p, t, seed + d, nxt, count=counts[d], d_n=cur_n)
+            cur, cur_n, k = nxt, counts[d].ptr, k ^ 1
+    return (counts, segs) if keep else counts
 
 
 def scan_u32(ctx: Context, a: np.ndarray, inclusive: bool = False) -> np.ndarray:

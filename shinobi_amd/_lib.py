@@ -114,7 +114,7 @@ SIGNATURES = {
     "sg_trace_shadow": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_uint32, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_void_p]),
     "sg_bounce": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_uint32,
-                            C.c_uint64, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_void_p]),
+                            C.c_void_p, C.c_uint64, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_void_p]),
     "sg_scan_u32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]),
     "sg_sort_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p]),
     "sg_malloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),

@@ -578,8 +578,8 @@ sg_status sg_trace_shadow(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint32_t
 }
 
 sg_status sg_bounce(sg_ctx* ctx, sg_scene* scene, const sg_rays* rays, const uint32_t* d_prim, const float* d_t,
-                    uint32_t n, uint64_t seed, const sg_rays_out* out, uint32_t* d_src, uint32_t* d_count,
-                    void* stream) {
+                    uint32_t n, const uint32_t* d_n, uint64_t seed, const sg_rays_out* out, uint32_t* d_src,
+                    uint32_t* d_count, void* stream) {
     if (!ctx || !scene || !rays || !d_prim || !d_t || !out || !d_count) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
     void* scratch = nullptr;
@@ -591,7 +591,7 @@ sg_status sg_bounce(sg_ctx* ctx, sg_scene* scene, const sg_rays* rays, const uin
         o.a[3 + i] = out->d[i];
     }
     o.a[6] = out->tmax;
-    launch_bounce(scene_soa(scene), ray_soa(rays), d_prim, d_t, n, seed, o, d_src, d_count, scratch,
+    launch_bounce(scene_soa(scene), ray_soa(rays), d_prim, d_t, n, d_n, seed, o, d_src, d_count, scratch,
                   (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return SG_OK;

@@ -136,7 +136,7 @@ void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t
 // receives the number of bounce rays.  scratch: bounce_scratch_bytes(n).
 size_t bounce_scratch_bytes(uint32_t n);
 void launch_bounce(const TriSoA& tris, const RaySoA& r, const uint32_t* prim, const float* t, uint32_t n,
-                   uint64_t seed, const RaySoAOut& out, uint32_t* src, uint32_t* d_count, void* scratch,
-                   hipStream_t st);
+                   const uint32_t* d_n, uint64_t seed, const RaySoAOut& out, uint32_t* src, uint32_t* d_count,
+                   void* scratch, hipStream_t st);
 
 }  // namespace sg

@@ -536,9 +536,10 @@ void launch_trace_items(const TraceItem* items, uint32_t n_items, uint32_t total
 // one-block scan of the counts, then each tile writes at its offset.
 constexpr int kBounceTile = kBlock;
 
-__global__ __launch_bounds__(kBlock) void k_bounce_count(const uint32_t* __restrict__ prim, uint32_t n,
-                                                         uint32_t* __restrict__ sums) {
+__global__ __launch_bounds__(kBlock) void k_bounce_count(const uint32_t* __restrict__ prim, uint32_t cap,
+                                                         const uint32_t* d_n, uint32_t* __restrict__ sums) {
     __shared__ uint32_t s_red[kBlock / 64];
+    const uint32_t n = d_n ? min(*d_n, cap) : cap;
     const uint32_t i = blockIdx.x * kBounceTile + threadIdx.x;
     const bool hit = i < n && prim[i] != kMiss;
     const uint32_t s = block_sum<kBlock / 64>(hit ? 1u : 0u, s_red);
@@ -570,10 +571,11 @@ __global__ __launch_bounds__(1024) void k_bounce_scan(uint32_t* sums, uint32_t t
 }
 
 __global__ __launch_bounds__(kBlock) void k_bounce(TriSoA tris, RaySoA rays, const uint32_t* __restrict__ prim,
-                                                   const float* __restrict__ tin, uint32_t n, uint64_t seed,
-                                                   RaySoAOut out, uint32_t* __restrict__ src,
+                                                   const float* __restrict__ tin, uint32_t cap, const uint32_t* d_n,
+                                                   uint64_t seed, RaySoAOut out, uint32_t* __restrict__ src,
                                                    const uint32_t* __restrict__ sums) {
     __shared__ uint32_t s_scan[kBlock / 64];
+    const uint32_t n = d_n ? min(*d_n, cap) : cap;
     const uint32_t i = blockIdx.x * kBounceTile + threadIdx.x;
     const bool hit = i < n && prim[i] != kMiss;
     uint32_t agg;
@@ -617,17 +619,17 @@ size_t bounce_scratch_bytes(uint32_t n) {
 }
 
 void launch_bounce(const TriSoA& tris, const RaySoA& r, const uint32_t* prim, const float* t, uint32_t n,
-                   uint64_t seed, const RaySoAOut& out, uint32_t* src, uint32_t* d_count, void* scratch,
-                   hipStream_t st) {
+                   const uint32_t* d_n, uint64_t seed, const RaySoAOut& out, uint32_t* src, uint32_t* d_count,
+                   void* scratch, hipStream_t st) {
     if (n == 0) {
         hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, d_count, 0u);
         return;
     }
     const uint32_t tiles = (n + kBounceTile - 1) / kBounceTile;
     uint32_t* sums = reinterpret_cast<uint32_t*>(scratch);
-    hipLaunchKernelGGL(k_bounce_count, dim3(tiles), dim3(kBlock), 0, st, prim, n, sums);
+    hipLaunchKernelGGL(k_bounce_count, dim3(tiles), dim3(kBlock), 0, st, prim, n, d_n, sums);
     hipLaunchKernelGGL(k_bounce_scan, dim3(1), dim3(1024), 0, st, sums, tiles, d_count);
-    hipLaunchKernelGGL(k_bounce, dim3(tiles), dim3(kBlock), 0, st, tris, r, prim, t, n, seed, out, src, sums);
+    hipLaunchKernelGGL(k_bounce, dim3(tiles), dim3(kBlock), 0, st, tris, r, prim, t, n, d_n, seed, out, src, sums);
 }
 
 }  // namespace sg

@@ -197,3 +197,29 @@ def test_shadow_any_hit(ctx, oracle, which):
     np.testing.assert_array_equal(ts < 0.9999, tc < 0.9999)
     occ = float((ts < 0.9999).mean())
     assert 0.01 < occ < 0.99, occ
+
+
+def test_cluster_paths_depth4(ctx, oracle):
+    """dense-cluster scene (500k tiny + 2k large triangles), diffuse paths of
+    four segments chained on the device (bounce counts never read back):
+    every segment's rays, ids and t equal the oracle's trace / bounce
+    composition, bit for bit"""
+    w = W.cluster(res_x=160, res_y=90)
+    og = oracle.build_twolevel(w.scene, **w.oracle)
+    scene, g = _build(ctx, w)
+    rb = sa.RayBatch(ctx, w.rays)
+    counts, segs = sa.trace_paths(ctx, scene, g, rb, depth=4, seed=w.bounce_seed, keep=True)
+    ctx.sync()
+    rays_o = w.rays
+    for d, (rays_d, prim_d, t_d, _) in enumerate(segs):
+        m = rays_o.shape[1]
+        if d:
+            assert int(counts[d - 1].download(np.uint32, 1)[0]) == m
+            np.testing.assert_array_equal(rays_d.download(m).view(np.uint32), rays_o.view(np.uint32))
+        po, to = og.trace(rays_o)
+        np.testing.assert_array_equal(prim_d.download(np.uint32, m), po, err_msg=f"segment {d}")
+        np.testing.assert_array_equal(t_d.download(np.float32, m).view(np.uint32), to.view(np.uint32))
+        if d < 3:
+            rays_o, _ = oracle.bounce(w.scene, rays_o, po, to, w.bounce_seed + d)
+            if rays_o.shape[1] == 0:
+                break
