@@ -129,7 +129,8 @@ def serving_bench(sa, W, ctx, seconds: float = 2.0, rate: float = 2000.0, burst:
     scene_of = rng.integers(0, 17, n_req)
 
     def run(mode: str, timed: bool):
-        srv = sa.Server(ctx, mode=mode, hbm_budget=3 << 30, max_requests=512, max_waves=3)
+        srv = sa.Server(ctx, mode=mode, hbm_budget=3 << 30, max_requests=512, max_waves=4,
+                        max_wave_rays=1 << 22)
         ids = [srv.add_grid(g) for _, g in grids]
         ids.append(srv.add_dynamic(frames[0], lam=5.0, lambda_leaf=2.0))
         stop = threading.Event()

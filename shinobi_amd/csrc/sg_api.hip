@@ -531,15 +531,11 @@ sg_status sg_grid_destroy(sg_grid* g) {
         hipStreamSynchronize(g->stream);
         hipFree(g->pool);
     }
-    if (g->tscratch) hipFree(g->tscratch);
     delete g;
     return SG_OK;
 }
 
 // ---------------------------------------------------------------- traversal
-// batches below this many rays walk in one pass
-constexpr uint32_t kMultiPassMin = 65536;
-
 static sg_status trace_common(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint32_t n, const uint32_t* d_count,
                               uint32_t* d_prim, float* d_t, uint32_t* d_stats, bool shadow, void* stream) {
     if (!ctx || !g || !rays || !d_prim || !d_t) return SG_ERR_INVALID_ARG;
@@ -547,21 +543,7 @@ static sg_status trace_common(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint
         if (!rays->o[i] || !rays->d[i]) return SG_ERR_INVALID_ARG;
     if (!rays->tmax) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
-    // large batches walk in compacting passes (queues in per-grid scratch;
-    // traces of one grid on different streams must not overlap in time)
-    void* scratch = nullptr;
-    if (!d_stats && n >= kMultiPassMin) {
-        const size_t need = trace_scratch_bytes(n);
-        if (g->tscratch_bytes < need) {
-            if (g->tscratch) HIPCHK(hipFree(g->tscratch));
-            g->tscratch = nullptr;
-            g->tscratch_bytes = 0;
-            if (hipMalloc(&g->tscratch, need) != hipSuccess) return SG_ERR_OUT_OF_MEMORY;
-            g->tscratch_bytes = need;
-        }
-        scratch = g->tscratch;
-    }
-    launch_trace(grid_view(g), ray_soa(rays), n, d_count, d_prim, d_t, d_stats, shadow, g->p.trace_block, scratch,
+    launch_trace(grid_view(g), ray_soa(rays), n, d_count, d_prim, d_t, d_stats, shadow, g->p.trace_block,
                  (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return SG_OK;

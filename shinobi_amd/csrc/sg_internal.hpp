@@ -49,9 +49,6 @@ struct sg_grid {
     size_t sort1_words = 0, sort2_words = 0;
     int key_bits1 = 1, key_bits2 = 1, passes1 = 1, passes2 = 1, rb = 8;
     hipStream_t stream = nullptr;
-    // queues of the compacting multi-pass traversal (grown on demand)
-    void* tscratch = nullptr;
-    size_t tscratch_bytes = 0;
 };
 
 #define HIPCHK(x)                                   \

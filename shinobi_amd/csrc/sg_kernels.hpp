@@ -126,12 +126,9 @@ void launch_trace_items(const TraceItem* items, uint32_t n_items, uint32_t total
                         bool shadow, hipStream_t st);
 void init_trace_items();
 // closest hit, or with shadow the any-hit traversal that stops at t < 0.9999
-// (block: threads per block, 64, 128 or 256).  With scratch
-// (trace_scratch_bytes(n) bytes) and no stats the walk runs in compacting
-// passes; without, in one pass.  Results are the same.
-size_t trace_scratch_bytes(uint32_t n);
+// (block: threads per block, 64, 128 or 256)
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim,
-                  float* t, uint32_t* stats, bool shadow, int block, void* scratch, hipStream_t st);
+                  float* t, uint32_t* stats, bool shadow, int block, hipStream_t st);
 // one cosine-weighted bounce per hit, compacted in primary order; *d_count
 // receives the number of bounce rays.  scratch: bounce_scratch_bytes(n).
 size_t bounce_scratch_bytes(uint32_t n);

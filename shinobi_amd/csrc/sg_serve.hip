@@ -436,7 +436,7 @@ sg_status launch_wave(sg_server* s, Slot& sl) {
         chunk += chunks_of(r->n);
         if (plain) {
             launch_trace(it.g, it.rays, it.n, nullptr, it.prim, it.t, nullptr, false, se.grid[gi]->p.trace_block,
-                         nullptr, sl.st);
+                         sl.st);
         } else {
             sl.pinned_items[k] = it;
         }

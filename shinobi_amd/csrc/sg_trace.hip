@@ -38,31 +38,109 @@ struct Ray {
 };
 
 // SimpleIntersector::MollerTrumboreIntersectionTest<.., true>,
-// src/Intersector/SimpleIntersector.hpp:50-89.  D is 1/rcp, the recomputed
-// direction (:58); strict < keeps the first of equal-t candidates.
+// src/Intersector/SimpleIntersector.hpp:50-89, for reference `it`: D is
+// 1/rcp, the recomputed direction (:58); strict < keeps the first of
+// equal-t candidates.
+SG_D void mt_test(const float4 q0, const float4 q1, const float q2, uint32_t it, const v3& o, const v3& D,
+                  float& t, uint32_t& best) {
+    const v3 v0 = mk(q0.x, q0.y, q0.z);
+    const v3 e1 = mk(q0.w, q1.x, q1.y);
+    const v3 e2 = mk(q1.z, q1.w, q2);
+    const v3 pv = cross(D, e2);
+    const float inv = 1.f / dot(e1, pv);
+    const v3 s = o - v0;
+    const float alpha = inv * dot(s, pv);
+    const v3 q = cross(s, e1);
+    const float beta = inv * dot(q, D);
+    const float dist = inv * dot(e2, q);
+    if (alpha >= 0.f && beta >= 0.f && alpha + beta <= 1.f && dist > kEps && dist < t) {
+        t = dist;
+        best = it;
+    }
+}
+
+SG_D void load_test(const uint32_t* __restrict__ indices, const float4* __restrict__ rec, uint32_t it, float4& q0,
+                    float4& q1, float& q2) {
+    const uint32_t tri = indices[it];
+    q0 = rec[4 * (size_t)tri];
+    q1 = rec[4 * (size_t)tri + 1];
+    q2 = reinterpret_cast<const float*>(rec + 4 * (size_t)tri + 2)[0];
+}
+
+constexpr uint32_t kUnroll = 4;     // tests whose loads are issued together
+constexpr uint32_t kCoopMin = 48;   // references from which the wave shares a cell's tests
+
+// The tests of references [b, e) in order.  Loads of kUnroll references are
+// issued before their tests (memory-level parallelism; the tests still run in
+// reference order).  A lane whose cell holds at least kCoopMin references
+// gets the whole wave: every active lane tests a strided share of the range
+// against that lane's ray and the wave keeps the minimum (t, reference),
+// which is what the ordered loop with strict < returns (the smallest
+// distance, the first reference among equal ones) -- the reference's
+// HybridIntersector idea (src/Intersector/HybridIntersector.hpp:472-704) for
+// dense leaves, with the same arithmetic per test.
 SG_D void intersect(const uint32_t* __restrict__ indices, const float4* __restrict__ rec, uint32_t b,
                     uint32_t e, Ray& r) {
-    for (uint32_t it = b; it != e; ++it) {
-        const uint32_t tri = indices[it];
-        const float4 q0 = rec[4 * (size_t)tri];
-        const float4 q1 = rec[4 * (size_t)tri + 1];
-        const float q2 = reinterpret_cast<const float*>(rec + 4 * (size_t)tri + 2)[0];
-        const v3 v0 = mk(q0.x, q0.y, q0.z);
-        const v3 e1 = mk(q0.w, q1.x, q1.y);
-        const v3 e2 = mk(q1.z, q1.w, q2);
-        const v3 pv = cross(r.D, e2);
-        const float inv = 1.f / dot(e1, pv);
-        const v3 s = r.o - v0;
-        const float alpha = inv * dot(s, pv);
-        const v3 q = cross(s, e1);
-        const float beta = inv * dot(q, r.D);
-        const float dist = inv * dot(e2, q);
-        if (alpha >= 0.f && beta >= 0.f && alpha + beta <= 1.f && dist > kEps && dist < r.t) {
-            r.t = dist;
-            r.best = it;
-        }
-    }
     r.tests += e - b;
+    const bool coop = e - b >= kCoopMin;
+    uint64_t big = __ballot(coop);
+    if (big) {
+        const uint64_t act = __ballot(true);
+        const uint32_t nact = (uint32_t)__popcll(act);
+        const uint32_t rank = (uint32_t)__popcll(act & lanemask_lt());
+        while (big) {
+            const int L = __ffsll((long long)big) - 1;
+            big &= big - 1;
+            const v3 ro = mk(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.o.x), L)),
+                             __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.o.y), L)),
+                             __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.o.z), L)));
+            const v3 rD = mk(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.D.x), L)),
+                             __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.D.y), L)),
+                             __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.D.z), L)));
+            const float t0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.t), L));
+            const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)b, L);
+            const uint32_t le = (uint32_t)__builtin_amdgcn_readlane((int)e, L);
+            float tl = t0;
+            uint32_t bl = kMiss;
+            for (uint32_t it = lb + rank; it < le; it += nact) {
+                float4 q0, q1;
+                float q2;
+                load_test(indices, rec, it, q0, q1, q2);
+                mt_test(q0, q1, q2, it, ro, rD, tl, bl);
+            }
+            // (distance bits, reference): positive distances order as
+            // unsigned integers, so the smallest key is the smallest t, then
+            // the first reference
+            const uint64_t key = bl == kMiss ? ~0ull : ((uint64_t)__float_as_uint(tl) << 32) | bl;
+            uint64_t m = ~0ull;
+            for (uint64_t a = act; a; a &= a - 1) {
+                const int j = __ffsll((long long)a) - 1;
+                const uint64_t kj = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(key >> 32), j) << 32) |
+                                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, j);
+                m = kj < m ? kj : m;
+            }
+            if (lane_id() == L && m != ~0ull) {
+                r.t = __uint_as_float((uint32_t)(m >> 32));
+                r.best = (uint32_t)m;
+            }
+        }
+        if (coop) return;
+    }
+    uint32_t it = b;
+    for (; it + kUnroll <= e; it += kUnroll) {
+        float4 q0[kUnroll], q1[kUnroll];
+        float q2[kUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; ++u) load_test(indices, rec, it + u, q0[u], q1[u], q2[u]);
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; ++u) mt_test(q0[u], q1[u], q2[u], it + u, r.o, r.D, r.t, r.best);
+    }
+    for (; it != e; ++it) {
+        float4 q0, q1;
+        float q2;
+        load_test(indices, rec, it, q0, q1, q2);
+        mt_test(q0, q1, q2, it, r.o, r.D, r.t, r.best);
+    }
 }
 
 // MIN_DIMENSION, src/Control/Algorithm/TwoLevelGridTraverser.h:633-634
@@ -307,161 +385,26 @@ __global__ __launch_bounds__(kBlock) void k_trace(GridView g, RaySoA rays, uint3
     trace_one<KIND, STATS, SHADOW>(g, rays, i, prim, tout, stats);
 }
 
-// ------------------------------------------------- compacting multi-pass walk
-// One ray per lane keeps neighbouring rays in one wave, but a wave runs as
-// long as its longest ray: on incoherent (bounce) rays the longest of 64
-// walks is several times the mean.  The walk therefore runs in passes: each
-// pass advances every live ray by at most `budget` top-level (uniform: cell)
-// steps, rays that finish store their result, and the others are appended,
-// wave by wave, to a compacted queue with their walk state, which the next
-// pass resumes with full waves.  The last pass has no budget.  Every ray
-// takes exactly the steps of the one-pass walk, so results are identical.
-struct WalkSave {
-    float t;
-    uint32_t best;
-    float cell[3], tmax[3];
-    float tEntry;
-    uint32_t guard;
-};
-
-template <int KIND, bool SHADOW>
-__global__ __launch_bounds__(kBlock) void k_trace_pass(GridView g, RaySoA rays, uint32_t cap, const uint32_t* d_n,
-                                                       uint32_t* __restrict__ prim, float* __restrict__ tout,
-                                                       const uint32_t* __restrict__ qin,
-                                                       const WalkSave* __restrict__ sin, const uint32_t* qin_n,
-                                                       uint32_t* __restrict__ qout, WalkSave* __restrict__ sout,
-                                                       uint32_t* qout_n, uint32_t budget) {
-    const uint32_t n = qin ? min(*qin_n, cap) : (d_n ? min(*d_n, cap) : cap);
-    if (blockIdx.x * blockDim.x >= n) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool more = false;
-    uint32_t id = 0;
-    Ray r;
-    Walk w;
-    if (i < n) {
-        id = qin ? qin[i] : i;
-        if (g.hdr->status & ~(ST_LEAF_RES | ST_NEG_EXTENT)) {
-            // a failed or overflowed build is never traversed: every ray misses
-            prim[id] = kMiss;
-            tout[id] = rays.a[6][id];
-        } else {
-            const GParams p = load_gparams(g.hdr);
-            r.o = mk(rays.a[0][id], rays.a[1][id], rays.a[2][id]);
-            const v3 d = mk(rays.a[3][id], rays.a[4][id], rays.a[5][id]);
-            r.rcp = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
-            r.D = mk(1.f / r.rcp.x, 1.f / r.rcp.y, 1.f / r.rcp.z);
-            r.tests = 0;
-            if (qin) {
-                const WalkSave sv = sin[i];
-                r.t = sv.t;
-                r.best = sv.best;
-                w.cell = mk(sv.cell[0], sv.cell[1], sv.cell[2]);
-                w.tMax = mk(sv.tmax[0], sv.tmax[1], sv.tmax[2]);
-                w.tEntry = sv.tEntry;
-                w.guard = sv.guard;
-                w.active = true;
-            } else {
-                r.t = rays.a[6][id];
-                r.best = kMiss;
-                if (KIND == 0) walk_init_uniform<SHADOW>(p, r, w);
-                else walk_init_twolevel<SHADOW>(p, r, w);
-            }
-            uint32_t s1 = 0, s2 = 0, s3 = 0;
-            if (KIND == 0) walk_uniform<SHADOW>(g, p, r, w, budget, s1);
-            else walk_twolevel<SHADOW>(g, p, r, w, budget, s1, s2, s3);
-            if (w.active && qout) {
-                more = true;
-            } else {
-                prim[id] = r.best == kMiss ? kMiss : g.indices[r.best];
-                tout[id] = r.t;
-            }
-        }
-    }
-    const uint64_t m = __ballot(more);
-    if (m) {
-        const int leader = __ffsll((long long)m) - 1;
-        uint32_t base = 0;
-        if (lane_id() == leader) base = atomicAdd(qout_n, (uint32_t)__popcll(m));
-        base = __shfl(base, leader, 64);
-        if (more) {
-            const uint32_t pos = base + (uint32_t)__popcll(m & lanemask_lt());
-            qout[pos] = id;
-            WalkSave sv;
-            sv.t = r.t;
-            sv.best = r.best;
-            sv.cell[0] = w.cell.x;
-            sv.cell[1] = w.cell.y;
-            sv.cell[2] = w.cell.z;
-            sv.tmax[0] = w.tMax.x;
-            sv.tmax[1] = w.tMax.y;
-            sv.tmax[2] = w.tMax.z;
-            sv.tEntry = w.tEntry;
-            sv.guard = w.guard;
-            sout[pos] = sv;
-        }
-    }
-}
-
 __global__ void k_set_counter(uint32_t* p) {
     if (threadIdx.x == 0) *p = 0u;
 }
 
-// step budgets of the passes (0: unlimited); top-level cells for the
-// two-level grid, cells for the uniform grid
-constexpr uint32_t kPassBudget[2][4] = {{32, 32, 64, 0}, {8, 8, 16, 0}};
-constexpr int kPasses = 4;
-
-size_t trace_scratch_bytes(uint32_t n) {
-    const size_t ib = ((size_t)n * sizeof(uint32_t) + 255) & ~(size_t)255;
-    const size_t sb = ((size_t)n * sizeof(WalkSave) + 255) & ~(size_t)255;
-    return 256 + 2 * ib + 2 * sb;
-}
-
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim, float* t,
-                  uint32_t* stats, bool shadow, int block, void* scratch, hipStream_t st) {
+                  uint32_t* stats, bool shadow, int block, hipStream_t st) {
     if (n == 0) return;
     // one ray per lane keeps the 64 lanes of a wave on neighbouring rays, so
     // coherent rays share cell, leaf and triangle loads
     const int B = (block == 64 || block == 128 || block == 256) ? block : kBlock;
     const dim3 grid((n + B - 1) / B), tb(B);
-    if (stats || !scratch) {
 #define SG_TRACE(K, S, H) hipLaunchKernelGGL((k_trace<K, S, H>), grid, tb, 0, st, g, r, n, d_n, prim, t, stats)
-        if (g.kind == 0) {
-            if (shadow) { if (stats) SG_TRACE(0, true, true); else SG_TRACE(0, false, true); }
-            else { if (stats) SG_TRACE(0, true, false); else SG_TRACE(0, false, false); }
-        } else {
-            if (shadow) { if (stats) SG_TRACE(1, true, true); else SG_TRACE(1, false, true); }
-            else { if (stats) SG_TRACE(1, true, false); else SG_TRACE(1, false, false); }
-        }
+    if (g.kind == 0) {
+        if (shadow) { if (stats) SG_TRACE(0, true, true); else SG_TRACE(0, false, true); }
+        else { if (stats) SG_TRACE(0, true, false); else SG_TRACE(0, false, false); }
+    } else {
+        if (shadow) { if (stats) SG_TRACE(1, true, true); else SG_TRACE(1, false, true); }
+        else { if (stats) SG_TRACE(1, true, false); else SG_TRACE(1, false, false); }
+    }
 #undef SG_TRACE
-        return;
-    }
-    // scratch: [2 counters][ids A][ids B][states A][states B]
-    char* sp = static_cast<char*>(scratch);
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(sp);
-    const size_t ib = ((size_t)n * sizeof(uint32_t) + 255) & ~(size_t)255;
-    const size_t sb = ((size_t)n * sizeof(WalkSave) + 255) & ~(size_t)255;
-    uint32_t* ids[2] = {reinterpret_cast<uint32_t*>(sp + 256), reinterpret_cast<uint32_t*>(sp + 256 + ib)};
-    WalkSave* sv[2] = {reinterpret_cast<WalkSave*>(sp + 256 + 2 * ib),
-                       reinterpret_cast<WalkSave*>(sp + 256 + 2 * ib + sb)};
-    const int kind = g.kind == 0 ? 0 : 1;
-    for (int pass = 0; pass < kPasses; ++pass) {
-        const uint32_t budget = kPassBudget[kind][pass] ? kPassBudget[kind][pass] : 0xFFFFFFFFu;
-        const bool last = pass == kPasses - 1;
-        const int in = (pass + 1) & 1, out = pass & 1;
-        const uint32_t* qin = pass ? ids[in] : nullptr;
-        const WalkSave* sin = pass ? sv[in] : nullptr;
-        const uint32_t* qin_n = pass ? cnt + in : nullptr;
-        uint32_t* qout = last ? nullptr : ids[out];
-        WalkSave* sout = last ? nullptr : sv[out];
-        uint32_t* qout_n = last ? nullptr : cnt + out;
-        if (qout_n) hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, st, qout_n);
-#define SG_PASS(K, H) hipLaunchKernelGGL((k_trace_pass<K, H>), grid, tb, 0, st, g, r, n, d_n, prim, t, qin, sin, \
-                                         qin_n, qout, sout, qout_n, budget)
-        if (kind == 0) { if (shadow) SG_PASS(0, true); else SG_PASS(0, false); }
-        else { if (shadow) SG_PASS(1, true); else SG_PASS(1, false); }
-#undef SG_PASS
-    }
 }
 
 // ------------------------------------------------- many batches, one launch
