@@ -304,6 +304,9 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true", help="skip the served-queries measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-scene measurement")
     ap.add_argument("--no-cluster", action="store_true", help="skip the dense-cluster path measurement")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a 64 Mi-element scan (k_scan_up reads 256 MiB, k_scan_down reads and writes "
+                         "256 MiB): the reference kernels for scaling FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -443,6 +446,14 @@ def main() -> int:
         "roofline_build": roofline_build,
         "cpu_baseline": cpu,
     }
+    if args.calibrate:
+        n_cal = 64 << 20
+        d_in = sa.DeviceBuffer(ctx, 4 * n_cal)
+        d_out = sa.DeviceBuffer(ctx, 4 * n_cal)
+        for _ in range(2):
+            sa.check(ctx._l.sg_scan_u32(ctx.h, d_in.ptr, d_out.ptr, n_cal, 0, ctx.stream), "sg_scan_u32")
+        ctx.sync()
+        del d_in, d_out
     if not args.no_cluster:
         out["cluster_paths"] = cluster_bench(sa, W, ctx)
     if not args.no_batched:

@@ -41,44 +41,68 @@ struct Ray {
 // src/Intersector/SimpleIntersector.hpp:50-89, for reference `it`: D is
 // 1/rcp, the recomputed direction (:58); strict < keeps the first of
 // equal-t candidates.
-SG_D void mt_test(const float4 q0, const float4 q1, const float q2, uint32_t it, const v3& o, const v3& D,
-                  float& t, uint32_t& best) {
+SG_D void mt_test(const float4 q0, const float4 q1, const float q2, uint32_t it, Ray& r) {
     const v3 v0 = mk(q0.x, q0.y, q0.z);
     const v3 e1 = mk(q0.w, q1.x, q1.y);
     const v3 e2 = mk(q1.z, q1.w, q2);
-    const v3 pv = cross(D, e2);
+    const v3 pv = cross(r.D, e2);
     const float inv = 1.f / dot(e1, pv);
-    const v3 s = o - v0;
+    const v3 s = r.o - v0;
     const float alpha = inv * dot(s, pv);
     const v3 q = cross(s, e1);
-    const float beta = inv * dot(q, D);
+    const float beta = inv * dot(q, r.D);
     const float dist = inv * dot(e2, q);
-    if (alpha >= 0.f && beta >= 0.f && alpha + beta <= 1.f && dist > kEps && dist < t) {
-        t = dist;
-        best = it;
+    if (alpha >= 0.f && beta >= 0.f && alpha + beta <= 1.f && dist > kEps && dist < r.t) {
+        r.t = dist;
+        r.best = it;
     }
 }
 
-SG_D void load_test(const uint32_t* __restrict__ indices, const float4* __restrict__ rec, uint32_t it, float4& q0,
-                    float4& q1, float& q2) {
-    const uint32_t tri = indices[it];
-    q0 = rec[4 * (size_t)tri];
-    q1 = rec[4 * (size_t)tri + 1];
-    q2 = reinterpret_cast<const float*>(rec + 4 * (size_t)tri + 2)[0];
+constexpr uint32_t kUnroll = 4;     // tests whose loads are issued together
+constexpr uint32_t kCoopMin = 64;   // references from which the wave shares a cell's tests
+
+// The wave tests the references [b, e) of lane L's cell against lane L's ray:
+// every active lane takes a strided share and the wave keeps the minimum
+// (t, reference), which is what the ordered loop with strict < returns (the
+// smallest distance, the first reference among equal ones).  Returns the
+// key (distance bits << 32 | reference), ~0 when nothing closer than t hit.
+__device__ __attribute__((noinline)) uint64_t coop_tests(const uint32_t* __restrict__ indices,
+                                                         const float4* __restrict__ rec, uint32_t lb, uint32_t le,
+                                                         float ox, float oy, float oz, float Dx, float Dy, float Dz,
+                                                         float t0, uint64_t act) {
+    const uint32_t nact = (uint32_t)__popcll(act);
+    const uint32_t rank = (uint32_t)__popcll(act & lanemask_lt());
+    Ray r;
+    r.o = mk(ox, oy, oz);
+    r.D = mk(Dx, Dy, Dz);
+    r.t = t0;
+    r.best = kMiss;
+    for (uint32_t it = lb + rank; it < le; it += nact) {
+        const uint32_t tri = indices[it];
+        mt_test(rec[4 * (size_t)tri], rec[4 * (size_t)tri + 1],
+                reinterpret_cast<const float*>(rec + 4 * (size_t)tri + 2)[0], it, r);
+    }
+    // positive distances order as unsigned integers: the smallest key is the
+    // smallest t, then the first reference
+    const uint64_t key = r.best == kMiss ? ~0ull : ((uint64_t)__float_as_uint(r.t) << 32) | r.best;
+    uint64_t m = ~0ull;
+    for (uint64_t a = act; a; a &= a - 1) {
+        const int j = __ffsll((long long)a) - 1;
+        const uint64_t kj = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(key >> 32), j) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, j);
+        m = kj < m ? kj : m;
+    }
+    return m;
 }
 
-constexpr uint32_t kUnroll = 4;     // tests whose loads are issued together
-constexpr uint32_t kCoopMin = 48;   // references from which the wave shares a cell's tests
+SG_D float lane_f(float x, int L) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), L)); }
 
 // The tests of references [b, e) in order.  Loads of kUnroll references are
 // issued before their tests (memory-level parallelism; the tests still run in
 // reference order).  A lane whose cell holds at least kCoopMin references
-// gets the whole wave: every active lane tests a strided share of the range
-// against that lane's ray and the wave keeps the minimum (t, reference),
-// which is what the ordered loop with strict < returns (the smallest
-// distance, the first reference among equal ones) -- the reference's
-// HybridIntersector idea (src/Intersector/HybridIntersector.hpp:472-704) for
-// dense leaves, with the same arithmetic per test.
+// gets the whole wave (coop_tests) -- the reference's HybridIntersector idea
+// (src/Intersector/HybridIntersector.hpp:472-704) for dense leaves, with the
+// same arithmetic per test.
 SG_D void intersect(const uint32_t* __restrict__ indices, const float4* __restrict__ rec, uint32_t b,
                     uint32_t e, Ray& r) {
     r.tests += e - b;
@@ -86,39 +110,13 @@ SG_D void intersect(const uint32_t* __restrict__ indices, const float4* __restri
     uint64_t big = __ballot(coop);
     if (big) {
         const uint64_t act = __ballot(true);
-        const uint32_t nact = (uint32_t)__popcll(act);
-        const uint32_t rank = (uint32_t)__popcll(act & lanemask_lt());
         while (big) {
             const int L = __ffsll((long long)big) - 1;
             big &= big - 1;
-            const v3 ro = mk(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.o.x), L)),
-                             __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.o.y), L)),
-                             __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.o.z), L)));
-            const v3 rD = mk(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.D.x), L)),
-                             __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.D.y), L)),
-                             __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.D.z), L)));
-            const float t0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.t), L));
-            const uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)b, L);
-            const uint32_t le = (uint32_t)__builtin_amdgcn_readlane((int)e, L);
-            float tl = t0;
-            uint32_t bl = kMiss;
-            for (uint32_t it = lb + rank; it < le; it += nact) {
-                float4 q0, q1;
-                float q2;
-                load_test(indices, rec, it, q0, q1, q2);
-                mt_test(q0, q1, q2, it, ro, rD, tl, bl);
-            }
-            // (distance bits, reference): positive distances order as
-            // unsigned integers, so the smallest key is the smallest t, then
-            // the first reference
-            const uint64_t key = bl == kMiss ? ~0ull : ((uint64_t)__float_as_uint(tl) << 32) | bl;
-            uint64_t m = ~0ull;
-            for (uint64_t a = act; a; a &= a - 1) {
-                const int j = __ffsll((long long)a) - 1;
-                const uint64_t kj = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(key >> 32), j) << 32) |
-                                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, j);
-                m = kj < m ? kj : m;
-            }
+            const uint64_t m = coop_tests(indices, rec, (uint32_t)__builtin_amdgcn_readlane((int)b, L),
+                                          (uint32_t)__builtin_amdgcn_readlane((int)e, L), lane_f(r.o.x, L),
+                                          lane_f(r.o.y, L), lane_f(r.o.z, L), lane_f(r.D.x, L), lane_f(r.D.y, L),
+                                          lane_f(r.D.z, L), lane_f(r.t, L), act);
             if (lane_id() == L && m != ~0ull) {
                 r.t = __uint_as_float((uint32_t)(m >> 32));
                 r.best = (uint32_t)m;
@@ -128,18 +126,24 @@ SG_D void intersect(const uint32_t* __restrict__ indices, const float4* __restri
     }
     uint32_t it = b;
     for (; it + kUnroll <= e; it += kUnroll) {
+        uint32_t tri[kUnroll];
         float4 q0[kUnroll], q1[kUnroll];
         float q2[kUnroll];
 #pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u) load_test(indices, rec, it + u, q0[u], q1[u], q2[u]);
+        for (uint32_t u = 0; u < kUnroll; ++u) tri[u] = indices[it + u];
 #pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u) mt_test(q0[u], q1[u], q2[u], it + u, r.o, r.D, r.t, r.best);
+        for (uint32_t u = 0; u < kUnroll; ++u) {
+            q0[u] = rec[4 * (size_t)tri[u]];
+            q1[u] = rec[4 * (size_t)tri[u] + 1];
+            q2[u] = reinterpret_cast<const float*>(rec + 4 * (size_t)tri[u] + 2)[0];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; ++u) mt_test(q0[u], q1[u], q2[u], it + u, r);
     }
     for (; it != e; ++it) {
-        float4 q0, q1;
-        float q2;
-        load_test(indices, rec, it, q0, q1, q2);
-        mt_test(q0, q1, q2, it, r.o, r.D, r.t, r.best);
+        const uint32_t tri = indices[it];
+        mt_test(rec[4 * (size_t)tri], rec[4 * (size_t)tri + 1],
+                reinterpret_cast<const float*>(rec + 4 * (size_t)tri + 2)[0], it, r);
     }
 }
 
