@@ -111,27 +111,10 @@ __global__ __launch_bounds__(kBlock) void k_bounds(TriSoA t, uint32_t n, uint32_
 }
 
 // --------------------------------------------------------------- params
-// Resolution and cell sizes from the reduced bounds, once (one thread):
-// GridBuilder::computeResolution (GridBuilder.h:41-62), the two-level top
-// resolution (FastTwoLevelGridBuilder.h:87-102) and the cell sizes of
-// FastSimpleGridBuilder.h:114-117 / SimpleGrid.h:99-107.  One block: first
-// the bounds partials of k_bounds' nb blocks are folded into the header.
-__global__ __launch_bounds__(kBlock) void k_params(GridHeader* h, BuildArgs a, const uint32_t* __restrict__ part,
-                                                   uint32_t nb, int derive) {
-    __shared__ uint32_t s_red[6][kBlock / 64];
-    uint32_t k[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-    for (uint32_t b = threadIdx.x; b < nb; b += kBlock)
-        for (int i = 0; i < 6; ++i) k[i] = i < 3 ? min(k[i], part[6 * b + i]) : max(k[i], part[6 * b + i]);
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1)
-        for (int i = 0; i < 6; ++i) {
-            const uint32_t o = (uint32_t)__shfl_xor(k[i], d, 64);
-            k[i] = i < 3 ? min(k[i], o) : max(k[i], o);
-        }
-    if (lane_id() == 0)
-        for (int i = 0; i < 6; ++i) s_red[i][wave_id()] = k[i];
-    __syncthreads();
-    if (threadIdx.x != 0) return;
+// thread 0 of k_params: the folded bounds keys, then (derive) resolution,
+// cell sizes and status; s_crcp receives the traversal cell_rcp
+SG_D void fold_params(GridHeader* h, const BuildArgs& a, const uint32_t (*s_red)[kBlock / 64], int derive,
+                      float* s_crcp) {
     for (int i = 0; i < 6; ++i) {
         uint32_t r = s_red[i][0];
         for (int w = 1; w < kBlock / 64; ++w) r = i < 3 ? min(r, s_red[i][w]) : max(r, s_red[i][w]);
@@ -158,9 +141,41 @@ __global__ __launch_bounds__(kBlock) void k_params(GridHeader* h, BuildArgs a, c
         h->cs[i] = ok ? cs[i] : 0.f;
         h->rcp_build[i] = ok ? rcp[i] : 0.f;
         h->cell_rcp[i] = ok ? crcp[i] : 0.f;
+        s_crcp[i] = h->cell_rcp[i];
     }
     h->C1 = bad ? 0u : (uint32_t)C;
     if (bad) h->status |= bad;
+}
+
+// Resolution and cell sizes from the reduced bounds, once (one thread):
+// GridBuilder::computeResolution (GridBuilder.h:41-62), the two-level top
+// resolution (FastTwoLevelGridBuilder.h:87-102) and the cell sizes of
+// FastSimpleGridBuilder.h:114-117 / SimpleGrid.h:99-107.  One block: first
+// the bounds partials of k_bounds' nb blocks are folded into the header;
+// with derive the block then fills the leaf cell size table.
+__global__ __launch_bounds__(kBlock) void k_params(GridHeader* h, BuildArgs a, const uint32_t* __restrict__ part,
+                                                   uint32_t nb, int derive) {
+    __shared__ uint32_t s_red[6][kBlock / 64];
+    uint32_t k[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    for (uint32_t b = threadIdx.x; b < nb; b += kBlock)
+        for (int i = 0; i < 6; ++i) k[i] = i < 3 ? min(k[i], part[6 * b + i]) : max(k[i], part[6 * b + i]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        for (int i = 0; i < 6; ++i) {
+            const uint32_t o = (uint32_t)__shfl_xor(k[i], d, 64);
+            k[i] = i < 3 ? min(k[i], o) : max(k[i], o);
+        }
+    if (lane_id() == 0)
+        for (int i = 0; i < 6; ++i) s_red[i][wave_id()] = k[i];
+    __syncthreads();
+    __shared__ float s_crcp[3];
+    if (threadIdx.x == 0) fold_params(h, a, s_red, derive, s_crcp);
+    if (!derive) return;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 3u * 256u; i += kBlock) {
+        const uint32_t d = i >> 8, c = i & 255u;
+        h->leaf_size[d][c] = 1.f / (s_crcp[d] * (float)c);
+    }
 }
 
 // grid parameters as every emission lane needs them (uniform loads)

@@ -47,6 +47,11 @@ struct alignas(256) GridHeader {
     uint32_t need_P1, need_C2, need_P2;  // true totals (exceed cap_* on overflow)
     uint32_t status;
     uint32_t leaf_sort;  // accepted leaf pairs the global sort must order (0: all tiles sorted in LDS)
+    // leaf cell size 1/(cell_rcp[d]*c) along d for a top cell of leaf
+    // resolution c: the subCellSize of the two-level traversal
+    // (src/Control/Algorithm/TwoLevelGridTraverser.h:499-510), computed once
+    // per build instead of three divisions per entered top cell
+    float leaf_size[3][256];
 };
 
 // Build parameters passed by value to the kernels.

@@ -76,4 +76,44 @@ SG_D uint32_t block_prefix_of(const uint32_t* __restrict__ sums, uint32_t b, uin
     return block_sum<NW>(acc, lds);
 }
 
+// ------------------------------------------- correctly rounded 1/x and a/b
+// Both return exactly what IEEE division returns, in fewer instructions
+// than the compiler's general division sequence (div_scale, rcp, five
+// fused multiply-adds, div_fmas, div_fixup).
+//
+// rcp_rn: v_rcp_f32 (within 1 ulp) refined by one Newton step whose
+// residual 1 - x*r is exact in a fused multiply-add; the step then rounds
+// to the correctly rounded reciprocal (P. Markstein, "Computation of
+// elementary functions on the IBM RISC System/6000 processor", IBM J. Res.
+// Dev. 34(1), 1990).  Checked against 1.f/x on gfx950 for all 2^32 inputs:
+// equal for every |x| in [2^-126, 2^126); the rest (zero, subnormal, huge,
+// inf, NaN) takes the IEEE division.
+SG_D bool rcp_fast_range(float x) { return fabsf(x) >= 0x1p-126f && fabsf(x) < 0x1p126f; }
+
+SG_D float rcp_rn(float x) {
+    if (rcp_fast_range(x)) {
+        const float r = __builtin_amdgcn_rcpf(x);
+        return fmaf(fmaf(-x, r, 1.f), r, r);
+    }
+    return 1.f / x;
+}
+
+// div_rn: a/b given y = 1/b correctly rounded: q = a*y is within one ulp
+// of a/b, the remainder a - b*q is exact in a fused multiply-add, and
+// q + remainder*y rounded once is a/b correctly rounded (Markstein 1990).
+// Valid while nothing under- or overflows: b and a nonzero have binary
+// exponents in [-60, 60] (checked on gfx950 against a/b for every b over
+// 1.3e10 pairs); other operands take the IEEE division.  For a = +-0 the
+// quotient is q = a*y itself (the correction step would turn -0 into +0).
+SG_D bool div_fast_range(float x) { return fabsf(x) >= 0x1p-60f && fabsf(x) < 0x1p61f; }
+
+SG_D float div_rn(float a, float b, float y, bool b_ok) {
+    if (b_ok && (a == 0.f || div_fast_range(a))) {
+        const float q = a * y;
+        const float f = fmaf(fmaf(-b, q, a), y, q);
+        return a == 0.f ? q : f;
+    }
+    return a / b;
+}
+
 }  // namespace sg

@@ -31,11 +31,18 @@ void launch_prep_tris(const TriSoA& t, uint32_t n, hipStream_t st) {
 
 // ---------------------------------------------------------------- traversal
 struct Ray {
-    v3 o, D, rcp;
+    v3 o, D, rcp;   // D = 1/rcp, the correctly rounded reciprocal of each rcp component
     float t;
     uint32_t best;
     uint32_t tests;
+    bool fast;      // every rcp component is a div_rn divisor (D serves as its reciprocal)
 };
+
+// a / rcp per component, as the reference's divisions by rayDirRCP
+SG_D v3 div_rcp(float a, const Ray& r) {
+    return mk(div_rn(a, r.rcp.x, r.D.x, r.fast), div_rn(a, r.rcp.y, r.D.y, r.fast),
+              div_rn(a, r.rcp.z, r.D.z, r.fast));
+}
 
 // SimpleIntersector::MollerTrumboreIntersectionTest<.., true>,
 // src/Intersector/SimpleIntersector.hpp:50-89, for reference `it`: D is
@@ -46,7 +53,7 @@ SG_D void mt_test(const float4 q0, const float4 q1, const float q2, uint32_t it,
     const v3 e1 = mk(q0.w, q1.x, q1.y);
     const v3 e2 = mk(q1.z, q1.w, q2);
     const v3 pv = cross(r.D, e2);
-    const float inv = 1.f / dot(e1, pv);
+    const float inv = rcp_rn(dot(e1, pv));
     const v3 s = r.o - v0;
     const float alpha = inv * dot(s, pv);
     const v3 q = cross(s, e1);
@@ -147,14 +154,46 @@ SG_D void intersect(const uint32_t* __restrict__ indices, const float4* __restri
     }
 }
 
-// MIN_DIMENSION, src/Control/Algorithm/TwoLevelGridTraverser.h:633-634
-SG_D int min_dim(v3 t) { return (t.x < t.y) ? ((t.x < t.z) ? 0 : 2) : ((t.y < t.z) ? 1 : 2); }
+// One DDA step along MIN_DIMENSION (src/Control/Algorithm/TwoLevelGridTraverser.h:633-634,
+// SimpleGridTraverser.h:378-398): the axis d with the smallest tMax (first of
+// x, y, z on ties, z when a comparison is unordered) advances cell[d] by
+// sign(rcp[d]) (-1 for rcp <= 0 or NaN) and tMax[d] by size[d]*|rcp[d]|; the
+// walk ends when cell[d] reaches res[d] or -1 (|difference| < 0.1).  The
+// axis is kept as two lane masks and every update is a select, so the step
+// has no branches; values are those of the per-axis scalar updates.
+struct Step {
+    bool x, y;     // d == 0, d == 1 (else d == 2)
+};
+
+SG_D Step min_axis(v3 t) {
+    const bool xy = t.x < t.y;
+    return Step{xy && t.x < t.z, !xy && t.y < t.z};
+}
+
+SG_D float sel(Step a, float x, float y, float z) { return a.x ? x : (a.y ? y : z); }
+
+// returns false when the step left the grid (reference: cell == res or -1)
+SG_D bool dda_step(Step a, v3& cell, v3& tMax, v3 rcp, v3 size, v3 resf) {
+    const float rc = sel(a, rcp.x, rcp.y, rcp.z);
+    const float inc = rc > 0.f ? 1.f : -1.f;
+    const float dt = sel(a, size.x, size.y, size.z) * fabsf(rc);
+    const bool z = !a.x && !a.y;
+    cell.x = a.x ? cell.x + inc : cell.x;
+    cell.y = a.y ? cell.y + inc : cell.y;
+    cell.z = z ? cell.z + inc : cell.z;
+    tMax.x = a.x ? tMax.x + dt : tMax.x;
+    tMax.y = a.y ? tMax.y + dt : tMax.y;
+    tMax.z = z ? tMax.z + dt : tMax.z;
+    const float cd = sel(a, cell.x, cell.y, cell.z);
+    const float rd = sel(a, resf.x, resf.y, resf.z);
+    return !(fabsf(cd - rd) < 0.1f || fabsf(cd + 1.f) < 0.1f);
+}
 
 // tex3D clamp addressing on integral coordinates
 SG_D uint32_t clamp_idx(float c, int res) { return (uint32_t)fminf(fmaxf(c, 0.f), (float)(res - 1)); }
 
 struct GParams {
-    v3 bmin, bmax, cs, crcp;
+    v3 bmin, bmax, cs, crcp, resf;
     int res[3];
     uint32_t C2;
 };
@@ -168,6 +207,7 @@ SG_D GParams load_gparams(const GridHeader* h) {
     p.res[0] = h->res[0];
     p.res[1] = h->res[1];
     p.res[2] = h->res[2];
+    p.resf = mk((float)p.res[0], (float)p.res[1], (float)p.res[2]);
     p.C2 = h->C2;
     return p;
 }
@@ -186,11 +226,9 @@ SG_D bool traverse_init(const GParams& p, const Ray& r, float& tEntry, v3& cell,
     v3 pt;
     if (TWO) {
         tEntry = fmaxf(0.f, tEntry);
-        const float te = tEntry + kEps;
-        pt = r.o + mk(te, te, te) / r.rcp;
+        pt = r.o + div_rcp(tEntry + kEps, r);
     } else {
-        const float te = tEntry + kEps;
-        pt = (tEntry >= 0.f) ? r.o + mk(te, te, te) / r.rcp : r.o;
+        pt = (tEntry >= 0.f) ? r.o + div_rcp(tEntry + kEps, r) : r.o;
     }
     cell = (pt - p.bmin) * p.crcp;
     cell = mk(floorf(cell.x), floorf(cell.y), floorf(cell.z));
@@ -246,12 +284,7 @@ SG_D void walk_uniform(const GridView& g, const GParams& p, Ray& r, Walk& w, uin
             break;
         }
         ++steps;
-        const int d = min_dim(w.tMax);
-        const float rc = comp(r.rcp, d);
-        setc(w.cell, d, comp(w.cell, d) + (rc > 0.f ? 1.f : -1.f));
-        setc(w.tMax, d, comp(w.tMax, d) + comp(p.cs, d) * fabsf(rc));
-        const float cd = comp(w.cell, d);
-        w.active = !(fabsf(cd - (float)p.res[d]) < 0.1f || fabsf(cd + 1.f) < 0.1f);
+        w.active = dda_step(min_axis(w.tMax), w.cell, w.tMax, r.rcp, p.cs, p.resf);
     }
 }
 
@@ -286,13 +319,13 @@ SG_D void walk_twolevel(const GridView& g, const GParams& p, Ray& r, Walk& w, ui
         if (top.y & kNotEmpty) {
             ++entered;
             const float tEntry = w.tEntry;
-            const v3 ep = r.o + mk(tEntry, tEntry, tEntry) / r.rcp;
+            const v3 ep = r.o + div_rcp(tEntry, r);
             const uint32_t c0 = top.y & 0xFFu, c1 = (top.y >> 8) & 0xFFu, c2 = (top.y >> 16) & 0xFFu;
             const v3 cres = mk((float)c0, (float)c1, (float)c2);
             const v3 srcp = p.crcp * cres;
             const v3 mb = cell * p.cs + p.bmin;
             v3 sc = mk(floorf((ep.x - mb.x) * srcp.x), floorf((ep.y - mb.y) * srcp.y), floorf((ep.z - mb.z) * srcp.z));
-            const v3 scs = mk(1.f / srcp.x, 1.f / srcp.y, 1.f / srcp.z);
+            const v3 scs = mk(g.hdr->leaf_size[0][c0], g.hdr->leaf_size[1][c1], g.hdr->leaf_size[2][c2]);
             sc = vmin(cres - mk(1.f, 1.f, 1.f), vmax(sc, mk(0.f, 0.f, 0.f)));
             v3 stMax;
             stMax.x = ((sc.x + (float)(r.rcp.x > 0.f ? 1 : 0)) * scs.x + mb.x - r.o.x) * r.rcp.x;
@@ -314,12 +347,7 @@ SG_D void walk_twolevel(const GridView& g, const GParams& p, Ray& r, Walk& w, ui
                     if (!flag2) break;
                 }
                 ++lsteps;
-                const int d = min_dim(stMax);
-                const float rc = comp(r.rcp, d);
-                setc(sc, d, comp(sc, d) + (rc > 0.f ? 1.f : -1.f));
-                setc(stMax, d, comp(stMax, d) + comp(scs, d) * fabsf(rc));
-                const float sd = comp(sc, d);
-                flag2 = !(fabsf(sd - comp(cres, d)) < 0.1f || fabsf(sd + 1.f) < 0.1f);
+                flag2 = dda_step(min_axis(stMax), sc, stMax, r.rcp, scs, cres);
             }
         }
         if (SHADOW ? r.t < kShadowT : !(r.t > w.tMax.x || r.t > w.tMax.y || r.t > w.tMax.z)) {
@@ -327,13 +355,9 @@ SG_D void walk_twolevel(const GridView& g, const GParams& p, Ray& r, Walk& w, ui
             break;
         }
         ++tsteps;
-        const int d = min_dim(w.tMax);
-        const float rc = comp(r.rcp, d);
-        w.tEntry = comp(w.tMax, d);
-        setc(w.cell, d, comp(w.cell, d) + (rc > 0.f ? 1.f : -1.f));
-        setc(w.tMax, d, comp(w.tMax, d) + comp(p.cs, d) * fabsf(rc));
-        const float cd = comp(w.cell, d);
-        w.active = !(fabsf(cd - (float)p.res[d]) < 0.1f || fabsf(cd + 1.f) < 0.1f);
+        const Step a = min_axis(w.tMax);
+        w.tEntry = sel(a, w.tMax.x, w.tMax.y, w.tMax.z);
+        w.active = dda_step(a, w.cell, w.tMax, r.rcp, p.cs, p.resf);
     }
 }
 
@@ -354,8 +378,9 @@ SG_D void trace_one(const GridView& g, const RaySoA& rays, uint32_t i, uint32_t*
     Ray r;
     r.o = mk(rays.a[0][i], rays.a[1][i], rays.a[2][i]);
     const v3 d = mk(rays.a[3][i], rays.a[4][i], rays.a[5][i]);
-    r.rcp = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
-    r.D = mk(1.f / r.rcp.x, 1.f / r.rcp.y, 1.f / r.rcp.z);
+    r.rcp = mk(rcp_rn(d.x), rcp_rn(d.y), rcp_rn(d.z));
+    r.D = mk(rcp_rn(r.rcp.x), rcp_rn(r.rcp.y), rcp_rn(r.rcp.z));
+    r.fast = div_fast_range(r.rcp.x) && div_fast_range(r.rcp.y) && div_fast_range(r.rcp.z);
     r.t = rays.a[6][i];
     r.best = kMiss;
     r.tests = 0;
