@@ -65,7 +65,6 @@ SG_D void mt_test(const float4 q0, const float4 q1, const float q2, uint32_t it,
     }
 }
 
-constexpr uint32_t kUnroll = 4;     // tests whose loads are issued together
 constexpr uint32_t kCoopMin = 64;   // references from which the wave shares a cell's tests
 
 // The wave tests the references [b, e) of lane L's cell against lane L's ray:
@@ -104,55 +103,6 @@ __device__ __attribute__((noinline)) uint64_t coop_tests(const uint32_t* __restr
 
 SG_D float lane_f(float x, int L) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), L)); }
 
-// The tests of references [b, e) in order.  Loads of kUnroll references are
-// issued before their tests (memory-level parallelism; the tests still run in
-// reference order).  A lane whose cell holds at least kCoopMin references
-// gets the whole wave (coop_tests) -- the reference's HybridIntersector idea
-// (src/Intersector/HybridIntersector.hpp:472-704) for dense leaves, with the
-// same arithmetic per test.
-SG_D void intersect(const uint32_t* __restrict__ indices, const float4* __restrict__ rec, uint32_t b,
-                    uint32_t e, Ray& r) {
-    r.tests += e - b;
-    const bool coop = e - b >= kCoopMin;
-    uint64_t big = __ballot(coop);
-    if (big) {
-        const uint64_t act = __ballot(true);
-        while (big) {
-            const int L = __ffsll((long long)big) - 1;
-            big &= big - 1;
-            const uint64_t m = coop_tests(indices, rec, (uint32_t)__builtin_amdgcn_readlane((int)b, L),
-                                          (uint32_t)__builtin_amdgcn_readlane((int)e, L), lane_f(r.o.x, L),
-                                          lane_f(r.o.y, L), lane_f(r.o.z, L), lane_f(r.D.x, L), lane_f(r.D.y, L),
-                                          lane_f(r.D.z, L), lane_f(r.t, L), act);
-            if (lane_id() == L && m != ~0ull) {
-                r.t = __uint_as_float((uint32_t)(m >> 32));
-                r.best = (uint32_t)m;
-            }
-        }
-        if (coop) return;
-    }
-    uint32_t it = b;
-    for (; it + kUnroll <= e; it += kUnroll) {
-        uint32_t tri[kUnroll];
-        float4 q0[kUnroll], q1[kUnroll];
-        float q2[kUnroll];
-#pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u) tri[u] = indices[it + u];
-#pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u) {
-            q0[u] = rec[4 * (size_t)tri[u]];
-            q1[u] = rec[4 * (size_t)tri[u] + 1];
-            q2[u] = reinterpret_cast<const float*>(rec + 4 * (size_t)tri[u] + 2)[0];
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u) mt_test(q0[u], q1[u], q2[u], it + u, r);
-    }
-    for (; it != e; ++it) {
-        const uint32_t tri = indices[it];
-        mt_test(rec[4 * (size_t)tri], rec[4 * (size_t)tri + 1],
-                reinterpret_cast<const float*>(rec + 4 * (size_t)tri + 2)[0], it, r);
-    }
-}
 
 // One DDA step along MIN_DIMENSION (src/Control/Algorithm/TwoLevelGridTraverser.h:633-634,
 // SimpleGridTraverser.h:378-398): the axis d with the smallest tMax (first of
@@ -243,8 +193,7 @@ SG_D bool traverse_init(const GParams& p, const Ray& r, float& tEntry, v3& cell,
 // SimpleGridTraverser.h:436)
 constexpr float kShadowT = 0.9999f;
 
-// The grid walk of one ray, resumable at top-level (uniform: cell) steps:
-// cell, tMax, tEntry and the step guard are the whole state between steps.
+// The top-level (uniform: only) walk state of one ray.
 struct Walk {
     v3 cell, tMax;
     float tEntry;
@@ -266,25 +215,86 @@ SG_D void walk_init_uniform(const GParams& p, const Ray& r, Walk& w) {
     w.active = (SHADOW ? r.t >= kShadowT : r.t > 0.f) && traverse_init<false>(p, r, w.tEntry, w.cell, w.tMax);
 }
 
-// at most `budget` cells of the walk
+// The walks below are flat loops: in every iteration each lane either
+// tests one reference of its current cell or, with the cell done, takes one
+// traversal step (t check, DDA step, next cell's range), and the wave keeps
+// iterating while any lane is busy.  Per lane this is the reference's
+// sequence of operations (nested cell and test loops); for the wave it
+// bounds the iteration count by its busiest lane instead of summing, level
+// by level, the longest loop of any lane, which is what a nested loop costs
+// when the 64 rays of a wave sit in cells of different sizes.  The index of
+// the next reference is loaded one test ahead, so a test waits for one
+// record load.  Lanes whose current cell holds at least kCoopMin references
+// get the whole wave for it (coop_tests).
+struct Refs {
+    uint32_t it, e;    // references [it, e) of the current cell not yet tested
+    uint32_t next;     // indices[it], loaded one test ahead
+};
+
+SG_D void refs_begin(const GridView& g, Refs& q, uint2 rg, Ray& r) {
+    q.it = rg.x;
+    q.e = rg.y;
+    r.tests += rg.y - rg.x;
+    if (q.it < q.e) q.next = g.indices[q.it];
+}
+
+// one test of each lane with references pending, after the wave has taken
+// over the dense cells
+SG_D void refs_step(const GridView& g, Refs& q, Ray& r) {
+    uint64_t big = __ballot(q.e - q.it >= kCoopMin && q.it < q.e);
+    if (big) {
+        const uint64_t act = __ballot(true);
+        while (big) {
+            const int L = __ffsll((long long)big) - 1;
+            big &= big - 1;
+            const uint64_t m = coop_tests(g.indices, g.rec, (uint32_t)__builtin_amdgcn_readlane((int)q.it, L),
+                                          (uint32_t)__builtin_amdgcn_readlane((int)q.e, L), lane_f(r.o.x, L),
+                                          lane_f(r.o.y, L), lane_f(r.o.z, L), lane_f(r.D.x, L), lane_f(r.D.y, L),
+                                          lane_f(r.D.z, L), lane_f(r.t, L), act);
+            if (lane_id() == L) {
+                if (m != ~0ull) {
+                    r.t = __uint_as_float((uint32_t)(m >> 32));
+                    r.best = (uint32_t)m;
+                }
+                q.it = q.e;
+            }
+        }
+    }
+    if (q.it < q.e) {
+        const uint32_t tri = q.next;
+        if (q.it + 1 < q.e) q.next = g.indices[q.it + 1];
+        mt_test(g.rec[4 * (size_t)tri], g.rec[4 * (size_t)tri + 1],
+                reinterpret_cast<const float*>(g.rec + 4 * (size_t)tri + 2)[0], q.it, r);
+        ++q.it;
+    }
+}
+
+SG_D uint2 load_cell(const GridView& g, const GParams& p, v3 c) {
+    const uint32_t k = clamp_idx(c.x, p.res[0]) + clamp_idx(c.y, p.res[1]) * (uint32_t)p.res[0] +
+                       clamp_idx(c.z, p.res[2]) * (uint32_t)p.res[0] * (uint32_t)p.res[1];
+    return *reinterpret_cast<const uint2*>(g.cells + 2 * (size_t)k);
+}
+
 template <bool SHADOW>
-SG_D void walk_uniform(const GridView& g, const GParams& p, Ray& r, Walk& w, uint32_t budget, uint32_t& steps) {
+SG_D void walk_uniform(const GridView& g, const GParams& p, Ray& r, Walk& w, uint32_t& steps) {
     const uint32_t cap = step_cap(p);
-    for (uint32_t it = 0; it < budget && w.active; ++it) {
-        if (!(w.guard++ < cap)) {
-            w.active = false;
-            break;
+    Refs q;
+    q.it = q.e = 0;
+    if (w.active && !(w.guard++ < cap)) w.active = false;
+    if (w.active) refs_begin(g, q, load_cell(g, p, w.cell), r);
+    while (__ballot(w.active)) {
+        refs_step(g, q, r);
+        if (w.active && q.it >= q.e) {
+            // the cell is done: t check, DDA step, next cell
+            if (SHADOW ? !(r.t >= kShadowT) : !(r.t > w.tMax.x || r.t > w.tMax.y || r.t > w.tMax.z)) {
+                w.active = false;
+            } else {
+                ++steps;
+                w.active = dda_step(min_axis(w.tMax), w.cell, w.tMax, r.rcp, p.cs, p.resf);
+                if (w.active && !(w.guard++ < cap)) w.active = false;
+                if (w.active) refs_begin(g, q, load_cell(g, p, w.cell), r);
+            }
         }
-        const uint32_t k = clamp_idx(w.cell.x, p.res[0]) + clamp_idx(w.cell.y, p.res[1]) * (uint32_t)p.res[0] +
-                           clamp_idx(w.cell.z, p.res[2]) * (uint32_t)p.res[0] * (uint32_t)p.res[1];
-        const uint2 rg = *reinterpret_cast<const uint2*>(g.cells + 2 * (size_t)k);
-        intersect(g.indices, g.rec, rg.x, rg.y, r);
-        if (SHADOW ? !(r.t >= kShadowT) : !(r.t > w.tMax.x || r.t > w.tMax.y || r.t > w.tMax.z)) {
-            w.active = false;
-            break;
-        }
-        ++steps;
-        w.active = dda_step(min_axis(w.tMax), w.cell, w.tMax, r.rcp, p.cs, p.resf);
     }
 }
 
@@ -301,63 +311,92 @@ SG_D void walk_init_twolevel(const GParams& p, const Ray& r, Walk& w) {
     if (!traverse_init<true>(p, r, w.tEntry, w.cell, w.tMax)) w.active = false;
 }
 
-// at most `budget` top cells of the walk (each entered top cell's leaf walk
-// runs to its end)
+// the leaf walk of one top cell (TwoLevelGridTraverser.h:489-528 set-up)
+struct Leaf {
+    v3 sc, stMax, scs, cres;
+    uint32_t base, c0, c1, icap, iguard;
+};
+
+SG_D uint2 load_leaf(const GridView& g, const GParams& p, const Leaf& L) {
+    const float lf = L.sc.x + (float)L.c0 * L.sc.y + (float)L.c0 * (float)L.c1 * L.sc.z;
+    const uint32_t li = L.base + f2u(lf);
+    return li < p.C2 ? *reinterpret_cast<const uint2*>(g.leaves + 2 * (size_t)li) : make_uint2(0u, 0u);
+}
+
+SG_D void leaf_enter(const GParams& p, const GridView& g, const Ray& r, const Walk& w, uint2 top, Leaf& L) {
+    const v3 ep = r.o + div_rcp(w.tEntry, r);
+    L.c0 = top.y & 0xFFu;
+    L.c1 = (top.y >> 8) & 0xFFu;
+    const uint32_t c2 = (top.y >> 16) & 0xFFu;
+    L.base = top.x;
+    L.cres = mk((float)L.c0, (float)L.c1, (float)c2);
+    const v3 srcp = p.crcp * L.cres;
+    const v3 mb = w.cell * p.cs + p.bmin;
+    v3 sc = mk(floorf((ep.x - mb.x) * srcp.x), floorf((ep.y - mb.y) * srcp.y), floorf((ep.z - mb.z) * srcp.z));
+    L.scs = mk(g.hdr->leaf_size[0][L.c0], g.hdr->leaf_size[1][L.c1], g.hdr->leaf_size[2][c2]);
+    sc = vmin(L.cres - mk(1.f, 1.f, 1.f), vmax(sc, mk(0.f, 0.f, 0.f)));
+    L.sc = sc;
+    L.stMax.x = ((sc.x + (float)(r.rcp.x > 0.f ? 1 : 0)) * L.scs.x + mb.x - r.o.x) * r.rcp.x;
+    L.stMax.y = ((sc.y + (float)(r.rcp.y > 0.f ? 1 : 0)) * L.scs.y + mb.y - r.o.y) * r.rcp.y;
+    L.stMax.z = ((sc.z + (float)(r.rcp.z > 0.f ? 1 : 0)) * L.scs.z + mb.z - r.o.z) * r.rcp.z;
+    L.icap = 2u * (L.c0 + L.c1 + c2) + 16u;
+    L.iguard = 1;   // the entry cell passed the guard
+}
+
 template <bool SHADOW>
-SG_D void walk_twolevel(const GridView& g, const GParams& p, Ray& r, Walk& w, uint32_t budget, uint32_t& lsteps,
-                        uint32_t& tsteps, uint32_t& entered) {
+SG_D void walk_twolevel(const GridView& g, const GParams& p, Ray& r, Walk& w, uint32_t& lsteps, uint32_t& tsteps,
+                        uint32_t& entered) {
     const uint32_t cap = step_cap(p);
-    for (uint32_t it = 0; it < budget && w.active; ++it) {
+    Refs q;
+    q.it = q.e = 0;
+    Leaf L;
+    bool in_leaf = false;
+    // enter the top cell at w.cell: a non-empty one starts its leaf walk
+    auto top_cell = [&]() {
         if (!(w.guard++ < cap)) {
             w.active = false;
-            break;
+            return;
         }
-        const v3 cell = w.cell;
-        const uint32_t k = clamp_idx(cell.x, p.res[0]) + clamp_idx(cell.y, p.res[1]) * (uint32_t)p.res[0] +
-                           clamp_idx(cell.z, p.res[2]) * (uint32_t)p.res[0] * (uint32_t)p.res[1];
-        const uint2 top = *reinterpret_cast<const uint2*>(g.cells + 2 * (size_t)k);
+        const uint2 top = load_cell(g, p, w.cell);
         if (top.y & kNotEmpty) {
             ++entered;
-            const float tEntry = w.tEntry;
-            const v3 ep = r.o + div_rcp(tEntry, r);
-            const uint32_t c0 = top.y & 0xFFu, c1 = (top.y >> 8) & 0xFFu, c2 = (top.y >> 16) & 0xFFu;
-            const v3 cres = mk((float)c0, (float)c1, (float)c2);
-            const v3 srcp = p.crcp * cres;
-            const v3 mb = cell * p.cs + p.bmin;
-            v3 sc = mk(floorf((ep.x - mb.x) * srcp.x), floorf((ep.y - mb.y) * srcp.y), floorf((ep.z - mb.z) * srcp.z));
-            const v3 scs = mk(g.hdr->leaf_size[0][c0], g.hdr->leaf_size[1][c1], g.hdr->leaf_size[2][c2]);
-            sc = vmin(cres - mk(1.f, 1.f, 1.f), vmax(sc, mk(0.f, 0.f, 0.f)));
-            v3 stMax;
-            stMax.x = ((sc.x + (float)(r.rcp.x > 0.f ? 1 : 0)) * scs.x + mb.x - r.o.x) * r.rcp.x;
-            stMax.y = ((sc.y + (float)(r.rcp.y > 0.f ? 1 : 0)) * scs.y + mb.y - r.o.y) * r.rcp.y;
-            stMax.z = ((sc.z + (float)(r.rcp.z > 0.f ? 1 : 0)) * scs.z + mb.z - r.o.z) * r.rcp.z;
-            bool flag2 = true;
-            const uint32_t icap = 2u * (c0 + c1 + c2) + 16u;
-            uint32_t iguard = 0;
-            while (flag2 && iguard++ < icap) {
-                const float lf = sc.x + (float)c0 * sc.y + (float)c0 * (float)c1 * sc.z;
-                const uint32_t li = top.x + f2u(lf);
-                uint2 rg = make_uint2(0u, 0u);
-                if (li < p.C2) rg = *reinterpret_cast<const uint2*>(g.leaves + 2 * (size_t)li);
-                intersect(g.indices, g.rec, rg.x, rg.y, r);
-                if (SHADOW) {
-                    if (r.t < kShadowT) break;
-                } else {
-                    flag2 = r.t > stMax.x || r.t > stMax.y || r.t > stMax.z;
-                    if (!flag2) break;
+            leaf_enter(p, g, r, w, top, L);
+            in_leaf = true;
+            refs_begin(g, q, load_leaf(g, p, L), r);
+        }
+    };
+    if (w.active) top_cell();
+    while (__ballot(w.active)) {
+        refs_step(g, q, r);
+        if (w.active && q.it >= q.e) {
+            bool top_next = !in_leaf;
+            if (in_leaf) {
+                // a leaf cell is done: t check, leaf DDA step, next leaf cell
+                bool go = SHADOW ? !(r.t < kShadowT) : (r.t > L.stMax.x || r.t > L.stMax.y || r.t > L.stMax.z);
+                if (go) {
+                    ++lsteps;
+                    go = dda_step(min_axis(L.stMax), L.sc, L.stMax, r.rcp, L.scs, L.cres) && L.iguard++ < L.icap;
                 }
-                ++lsteps;
-                flag2 = dda_step(min_axis(stMax), sc, stMax, r.rcp, scs, cres);
+                if (go) {
+                    refs_begin(g, q, load_leaf(g, p, L), r);
+                } else {
+                    in_leaf = false;
+                    top_next = true;
+                }
+            }
+            if (top_next) {
+                // the top cell is done: t check, top DDA step, next top cell
+                if (SHADOW ? r.t < kShadowT : !(r.t > w.tMax.x || r.t > w.tMax.y || r.t > w.tMax.z)) {
+                    w.active = false;
+                } else {
+                    ++tsteps;
+                    const Step a = min_axis(w.tMax);
+                    w.tEntry = sel(a, w.tMax.x, w.tMax.y, w.tMax.z);
+                    w.active = dda_step(a, w.cell, w.tMax, r.rcp, p.cs, p.resf);
+                    if (w.active) top_cell();
+                }
             }
         }
-        if (SHADOW ? r.t < kShadowT : !(r.t > w.tMax.x || r.t > w.tMax.y || r.t > w.tMax.z)) {
-            w.active = false;
-            break;
-        }
-        ++tsteps;
-        const Step a = min_axis(w.tMax);
-        w.tEntry = sel(a, w.tMax.x, w.tMax.y, w.tMax.z);
-        w.active = dda_step(a, w.cell, w.tMax, r.rcp, p.cs, p.resf);
     }
 }
 
@@ -388,10 +427,10 @@ SG_D void trace_one(const GridView& g, const RaySoA& rays, uint32_t i, uint32_t*
     Walk w;
     if (KIND == 0) {
         walk_init_uniform<SHADOW>(p, r, w);
-        walk_uniform<SHADOW>(g, p, r, w, 0xFFFFFFFFu, s1);
+        walk_uniform<SHADOW>(g, p, r, w, s1);
     } else {
         walk_init_twolevel<SHADOW>(p, r, w);
-        walk_twolevel<SHADOW>(g, p, r, w, 0xFFFFFFFFu, s1, s2, s3);
+        walk_twolevel<SHADOW>(g, p, r, w, s1, s2, s3);
     }
     prim[i] = r.best == kMiss ? kMiss : g.indices[r.best];
     tout[i] = r.t;
