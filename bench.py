@@ -222,6 +222,46 @@ def cluster_bench(sa, W, ctx, spp: int = 4, depth: int = 4, reps: int = 3) -> di
             "rays": best[2], "trace_ms": round(best[1], 3), "mrays_per_s": round(best[2] / best[1] / 1e3, 1)}
 
 
+def ruins_bench(sa, W, ctx, reps: int = 10) -> dict:
+    """The reference's own scene (scenes/ruins: 38,665 triangles, its
+    1024x1024 camera.cfg view): two-level rebuild with the reference
+    builder's defaults, primary trace, one diffuse bounce, bounce trace."""
+    golden = os.path.join(ROOT, "tests", "golden")
+    w = W.scene_file(os.path.join(golden, "ruins.obj.gz"), os.path.join(golden, "ruins_camera.cfg"))
+    scene = sa.FaceSoup(ctx, w.scene)
+    b = sa.FastTwoLevelGridBuilder(ctx, **w.build)
+    b.init(scene)
+    g = b.build()
+    info = g.info()
+    n = w.rays.shape[1]
+    rays = sa.RayBatch(ctx, w.rays)
+    prim, t = sa.DeviceBuffer(ctx, 4 * n), sa.DeviceBuffer(ctx, 4 * n)
+    out = sa.RayBatch(ctx, capacity=n)
+    bp, bt = sa.DeviceBuffer(ctx, 4 * n), sa.DeviceBuffer(ctx, 4 * n)
+    count = sa.DeviceBuffer(ctx, 4)
+    ev = [ctx.event() for _ in range(5)]
+    rows = []
+    for rep in range(reps + 2):
+        ev[0].record()
+        g.rebuild()
+        ev[1].record()
+        g.trace(rays, prim=prim, t=t)
+        ev[2].record()
+        sa.bounce(ctx, scene, rays, prim, t, w.bounce_seed, out, count=count)
+        ev[3].record()
+        g.trace(out, prim=bp, t=bt, d_count=count.ptr, n=n)
+        ev[4].record()
+        ctx.sync()
+        if rep >= 2:
+            rows.append([ev[i].elapsed_ms(ev[i + 1]) for i in range(4)])
+    m = int(count.download(np.uint32, 1)[0])
+    r = np.median(np.array(rows), axis=0)
+    return {"n_tris": int(w.scene.shape[1]), "primary_rays": n, "bounce_rays": m, "res_top": list(info.res),
+            "C1": info.C1, "P1": info.P1, "C2": info.C2, "P2": info.P2, "build_ms": round(float(r[0]), 4),
+            "trace_primary_ms": round(float(r[1]), 4), "trace_bounce_ms": round(float(r[3]), 4),
+            "mrays_per_s": round((n + m) / (r[1] + r[3]) / 1e3, 1)}
+
+
 def batched_bench(sa, ctx, n_scenes: int = 64, reps: int = 5) -> dict:
     """Batched multi-scene launch: 64 uniform-in-cube soups, N log-uniform in
     [1e3, 5e5] with edge scale 0.01*N^(-1/3), each with log-uniform
@@ -317,6 +357,7 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true", help="skip the served-queries measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-scene measurement")
     ap.add_argument("--no-cluster", action="store_true", help="skip the dense-cluster path measurement")
+    ap.add_argument("--no-ruins", action="store_true", help="skip the reference ruins scene measurement")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 64 Mi-element scan (k_scan_up reads 256 MiB, k_scan_down reads and writes "
                          "256 MiB): the reference kernels for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -475,6 +516,8 @@ def main() -> int:
             sa.check(ctx._l.sg_scan_u32(ctx.h, d_in.ptr, d_out.ptr, n_cal, 0, ctx.stream), "sg_scan_u32")
         ctx.sync()
         del d_in, d_out
+    if not args.no_ruins:
+        out["ruins"] = ruins_bench(sa, W, ctx)
     if not args.no_cluster:
         out["cluster_paths"] = cluster_bench(sa, W, ctx)
     if not args.no_batched:

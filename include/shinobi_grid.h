@@ -38,7 +38,8 @@ enum {
     SG_ERR_NEG_EXTENT = 8,      /* a negative leaf extent (reference corrupts)     */
     SG_ERR_TIMEOUT = 9,         /* a device-side wait exceeded its bound           */
     SG_ERR_BUDGET = 10,         /* request exceeds the serving HBM budget          */
-    SG_ERR_NO_DEVICE = 11       /* no HIP device                                   */
+    SG_ERR_NO_DEVICE = 11,      /* no HIP device                                   */
+    SG_ERR_IO = 12              /* a scene or camera file cannot be read           */
 };
 
 const char *sg_status_str(sg_status s);
@@ -331,6 +332,36 @@ sg_status sg_gen_camera_rays(const float eye[3], const float fwd[3], const float
                              uint32_t res_x, uint32_t res_y, float *const rays7[7]);
 /* rays with origins uniform in [lo,hi]^3 and directions uniform on the sphere */
 sg_status sg_gen_random_rays(uint64_t seed, uint32_t n, float lo, float hi, float *const rays7[7]);
+
+/* ---------------------------------------------------------- scene files --- */
+/* Wavefront OBJ geometry, replacing FWObject::read (src/Loaders/FWObjectReader.cpp:181-502)
+ * as far as it decides triangles: accepted / rejected lines, (float)strtod
+ * coordinates, 1-based strtol indices, fan triangulation (0,k,k+1) of polygons.
+ * Rejected lines are skipped and counted, as the reference skips them.
+ * SG_ERR_IO if the file cannot be read; SG_ERR_INVALID_SCENE if a face index
+ * lies outside the vertex list. Host-only; the triangles feed sg_scene_create
+ * (FaceSoup::upload, src/Control/Structure/FaceSoup.h:171-247). */
+typedef struct sg_obj sg_obj;
+typedef struct sg_obj_info {
+    uint32_t n_vertices;   /* accepted 'v' lines */
+    uint32_t n_faces;      /* accepted face lines */
+    uint32_t n_triangles;  /* after fan triangulation */
+    uint32_t n_bad_lines;  /* lines the reader rejects */
+} sg_obj_info;
+sg_status sg_obj_load(const char *path, sg_obj **out, sg_obj_info *info);
+/* triangle i = (v[a], v[b], v[c]) of its face into 9 host arrays of n_triangles floats */
+sg_status sg_obj_triangles(const sg_obj *obj, float *const soa9[9]);
+void sg_obj_destroy(sg_obj *obj);
+
+/* camera.cfg, replacing CameraManager::read (src/Loaders/CameraManager.cpp:93-165):
+ * position / orientation / up / rotation / resX / resY / FOV keyword lines;
+ * lines holding '#' are skipped. Feed sg_gen_camera_rays(position, orientation, up, fov_deg, res_x, res_y). */
+typedef struct sg_camera {
+    float position[3], orientation[3], up[3], rotation[3];
+    uint32_t res_x, res_y;
+    float fov_deg;
+} sg_camera;
+sg_status sg_camera_load(const char *path, sg_camera *out);
 
 #ifdef __cplusplus
 }

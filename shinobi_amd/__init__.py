@@ -21,6 +21,7 @@ library; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -35,7 +36,7 @@ __all__ = [
     "Context", "DeviceBuffer", "FaceSoup", "FastSimpleGridBuilder", "FastTwoLevelGridBuilder", "Grid",
     "GridInfo", "RayBatch", "ShinobiError", "MISS", "FLT_MAX", "gen_soup", "gen_heightfield",
     "gen_cluster", "gen_camera_rays", "gen_random_rays", "heightfield_height", "build_params",
-    "device_count", "lib",
+    "device_count", "lib", "load_obj", "load_camera",
 ]
 
 
@@ -687,6 +688,30 @@ def gen_camera_rays(eye, fwd, up, fov_deg: float, res_x: int, res_y: int) -> np.
     f3 = C.c_float * 3
     check(lib().sg_gen_camera_rays(f3(*eye), f3(*fwd), f3(*up), fov_deg, res_x, res_y, p), "sg_gen_camera_rays")
     return a
+
+
+def load_obj(path: str):
+    """Triangles of a Wavefront OBJ file as a (9, T) float32 SoA array (the
+    reference's FWObject reader, src/Loaders/FWObjectReader.cpp:181-502) and
+    the reader's counts {n_vertices, n_faces, n_triangles, n_bad_lines}."""
+    h = C.c_void_p()
+    info = L.ObjInfo()
+    check(lib().sg_obj_load(os.fsencode(path), C.byref(h), C.byref(info)), "sg_obj_load")
+    try:
+        a, p = _soa(info.n_triangles, 9)
+        check(lib().sg_obj_triangles(h, p), "sg_obj_triangles")
+    finally:
+        lib().sg_obj_destroy(h)
+    return a, {k: getattr(info, k) for k, _ in L.ObjInfo._fields_}
+
+
+def load_camera(path: str) -> dict:
+    """camera.cfg (src/Loaders/CameraManager.cpp:93-165): position,
+    orientation, up, rotation, res_x, res_y, fov_deg"""
+    cam = L.Camera()
+    check(lib().sg_camera_load(os.fsencode(path), C.byref(cam)), "sg_camera_load")
+    return {"position": tuple(cam.position), "orientation": tuple(cam.orientation), "up": tuple(cam.up),
+            "rotation": tuple(cam.rotation), "res_x": cam.res_x, "res_y": cam.res_y, "fov_deg": cam.fov_deg}
 
 
 def gen_random_rays(seed: int, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:

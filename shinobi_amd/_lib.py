@@ -83,6 +83,16 @@ class ServerConfig(C.Structure):
                 ("max_waves", C.c_uint32), ("max_wave_rays", C.c_uint32)]
 
 
+class ObjInfo(C.Structure):
+    _fields_ = [("n_vertices", C.c_uint32), ("n_faces", C.c_uint32), ("n_triangles", C.c_uint32),
+                ("n_bad_lines", C.c_uint32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("orientation", C.c_float * 3), ("up", C.c_float * 3),
+                ("rotation", C.c_float * 3), ("res_x", C.c_uint32), ("res_y", C.c_uint32), ("fov_deg", C.c_float)]
+
+
 class ServerStats(C.Structure):
     _fields_ = [("requests", C.c_uint64), ("rays", C.c_uint64), ("waves", C.c_uint64),
                 ("p50_us", C.c_double), ("p99_us", C.c_double), ("mean_us", C.c_double), ("max_us", C.c_double),
@@ -161,6 +171,10 @@ SIGNATURES = {
     "sg_gen_camera_rays": (C.c_int, [C.c_float * 3, C.c_float * 3, C.c_float * 3, C.c_float, C.c_uint32,
                                      C.c_uint32, Float7]),
     "sg_gen_random_rays": (C.c_int, [C.c_uint64, C.c_uint32, C.c_float, C.c_float, Float7]),
+    "sg_obj_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(ObjInfo)]),
+    "sg_obj_triangles": (C.c_int, [C.c_void_p, Float9]),
+    "sg_obj_destroy": (None, [C.c_void_p]),
+    "sg_camera_load": (C.c_int, [C.c_char_p, C.POINTER(Camera)]),
 }
 
 _lib = None

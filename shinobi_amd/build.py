@@ -21,7 +21,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libshinobi_grid.so")
 
 HIP_SOURCES = ["sg_build.hip", "sg_sort.hip", "sg_trace.hip", "sg_api.hip", "sg_serve.hip"]
-CXX_SOURCES = ["sg_gen.cpp"]
+CXX_SOURCES = ["sg_gen.cpp", "sg_obj.cpp"]
 
 ARCH = os.environ.get("SHINOBI_ARCH", "gfx950")
 

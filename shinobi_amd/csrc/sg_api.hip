@@ -43,6 +43,7 @@ const char* sg_status_str(sg_status s) {
         case SG_ERR_TIMEOUT: return "device-side wait exceeded its bound";
         case SG_ERR_BUDGET: return "request exceeds the HBM budget";
         case SG_ERR_NO_DEVICE: return "no HIP device";
+        case SG_ERR_IO: return "file cannot be read";
         default: return "unknown status";
     }
 }

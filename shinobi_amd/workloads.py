@@ -11,6 +11,9 @@ parameters), shared by the tests, the smoke check and bench.py.
   lambda 2 leaf, 16-reference leaf threshold.
 * ``cluster`` -- 500,000 tiny triangles in a sphere of radius 0.02 at
   (0.5,0.5,0.5) plus 2,000 large triangles in [0,10]^3 (seed 3); two-level.
+* ``scene_file`` -- an OBJ scene and its camera.cfg (the reference's ruins
+  scene: 38,665 triangles, 1024x1024 camera); two-level grid with the
+  reference builder's defaults (lambda 5 / 6 top, lambda 1.2 leaf).
 """
 from __future__ import annotations
 
@@ -18,7 +21,12 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import gen_camera_rays, gen_cluster, gen_heightfield, gen_soup, heightfield_height
+import gzip
+import os
+import shutil
+import tempfile
+
+from . import gen_camera_rays, gen_cluster, gen_heightfield, gen_soup, heightfield_height, load_camera, load_obj
 from ._lib import SG_GRID_TWO_LEVEL, SG_GRID_UNIFORM
 
 
@@ -89,3 +97,20 @@ def sphere_mesh(n_tris: int = 200_000, frame: int = 0, seed: int = 5) -> np.ndar
     t1 = np.concatenate([a, b, c]).reshape(9, -1)
     t2 = np.concatenate([a, c, e]).reshape(9, -1)
     return np.ascontiguousarray(np.concatenate([t1, t2], axis=1), np.float32)
+
+
+def scene_file(obj_path: str, camera_path: str, name: str = "ruins") -> Workload:
+    """An OBJ scene (plain or .gz) with the primary rays of its camera.cfg."""
+    if obj_path.endswith(".gz"):
+        with tempfile.TemporaryDirectory() as d:
+            plain = os.path.join(d, "scene.obj")
+            with gzip.open(obj_path, "rb") as f, open(plain, "wb") as g:
+                shutil.copyfileobj(f, g)
+            scene, _ = load_obj(plain)
+    else:
+        scene, _ = load_obj(obj_path)
+    cam = load_camera(camera_path)
+    rays = gen_camera_rays(cam["position"], cam["orientation"], cam["up"], cam["fov_deg"], cam["res_x"], cam["res_y"])
+    return Workload(name, SG_GRID_TWO_LEVEL, scene, rays,
+                    dict(lam=5.0, top_divisor=6, lambda_leaf=1.2, leaf_min_refs=16), bounce_seed=107,
+                    oracle=dict(lam=5.0, top_divisor=6, lambda_leaf=1.2, leaf_min_refs=16))
