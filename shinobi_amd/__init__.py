@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -61,11 +62,26 @@ class Context:
         self.h = h
         self.device = device
         self.stream = C.c_void_p()
+        self._owned = weakref.WeakSet()
         if own_stream:
             check(self._l.sg_stream_create(self.h, C.byref(self.stream)), "sg_stream_create")
 
+    def _own(self, obj) -> None:
+        self._owned.add(obj)
+
+    # objects that depend on others close first
+    _CLOSE_ORDER = ("Server", "Graph", "Batch", "Grid", "FaceSoup", "RayBatch", "DeviceBuffer", "PinnedArray",
+                    "Event")
+
     def close(self) -> None:
+        """Close every object still open on this context, then the context."""
         if getattr(self, "h", None):
+            rank = {n: i for i, n in enumerate(self._CLOSE_ORDER)}
+            for obj in sorted(list(self._owned), key=lambda o: rank.get(type(o).__name__, len(rank))):
+                try:
+                    obj.close()
+                except Exception:
+                    pass
             if self.stream:
                 self._l.sg_stream_sync(self.h, self.stream)
                 self._l.sg_stream_destroy(self.h, self.stream)
@@ -100,6 +116,7 @@ class Event:
         self.ctx = ctx
         self.h = C.c_void_p()
         check(ctx._l.sg_event_create(ctx.h, C.byref(self.h)), "sg_event_create")
+        ctx._own(self)
 
     def record(self, stream=None) -> None:
         check(self.ctx._l.sg_event_record(self.ctx.h, self.h, self.ctx.stream if stream is None else stream),
@@ -110,10 +127,14 @@ class Event:
         check(self.ctx._l.sg_event_elapsed_ms(self.ctx.h, self.h, end.h, C.byref(ms)), "sg_event_elapsed_ms")
         return float(ms.value)
 
+    def close(self) -> None:
+        if self.h and self.ctx.h:
+            self.ctx._l.sg_event_destroy(self.ctx.h, self.h)
+        self.h = C.c_void_p()
+
     def __del__(self):
         try:
-            if self.h:
-                self.ctx._l.sg_event_destroy(self.ctx.h, self.h)
+            self.close()
         except Exception:
             pass
 
@@ -126,6 +147,7 @@ class DeviceBuffer:
         self.nbytes = int(nbytes)
         self.ptr = C.c_void_p()
         check(ctx._l.sg_malloc(ctx.h, max(self.nbytes, 1), C.byref(self.ptr)), "sg_malloc")
+        ctx._own(self)
 
     @property
     def addr(self) -> int:
@@ -151,9 +173,11 @@ class DeviceBuffer:
         return out
 
     def free(self) -> None:
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             self.ctx._l.sg_free(self.ctx.h, self.ptr)
-            self.ptr = C.c_void_p()
+        self.ptr = C.c_void_p()
+
+    close = free
 
     def __del__(self):
         try:
@@ -171,13 +195,15 @@ class PinnedArray:
         n = int(np.prod(shape)) * dtype.itemsize
         self.ptr = C.c_void_p()
         check(ctx._l.sg_host_alloc(ctx.h, max(n, 1), C.byref(self.ptr)), "sg_host_alloc")
+        ctx._own(self)
         buf = (C.c_char * max(n, 1)).from_address(self.ptr.value)
         self.a = np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
 
     def close(self) -> None:
         if self.ptr:
             self.a = None
-            self.ctx._l.sg_host_free(self.ctx.h, self.ptr)
+            if self.ctx.h:
+                self.ctx._l.sg_host_free(self.ctx.h, self.ptr)
             self.ptr = C.c_void_p()
 
     def __del__(self):
@@ -233,6 +259,7 @@ class FaceSoup:
         h = C.c_void_p()
         check(ctx._l.sg_scene_create(ctx.h, ptrs, self.n, 0, ctx.stream, C.byref(h)), "sg_scene_create")
         self.h = h
+        ctx._own(self)
 
     def upload_vertices(self, soa9: np.ndarray) -> None:
         """FaceSoup::uploadVertices: new positions, same topology."""
@@ -307,6 +334,7 @@ class Grid:
 
     def __init__(self, ctx: Context, scene: FaceSoup, h: C.c_void_p, params: L.BuildParams):
         self.ctx, self.scene, self.h, self.params = ctx, scene, h, params
+        ctx._own(self)
 
     def info(self) -> GridInfo:
         gi = L.GridInfo()
@@ -478,6 +506,7 @@ class Batch:
         self.ctx = ctx
         self.h = C.c_void_p()
         check(ctx._l.sg_batch_create(ctx.h, max_items, C.byref(self.h)), "sg_batch_create")
+        ctx._own(self)
         self._keep: list = []
 
     def clear(self) -> None:
@@ -516,6 +545,7 @@ class Graph:
     def __init__(self, ctx: Context):
         self.ctx = ctx
         self.h = C.c_void_p()
+        ctx._own(self)
 
     @classmethod
     def capture(cls, ctx: Context) -> "Graph":
@@ -586,6 +616,7 @@ class Server:
         cfg.max_wave_rays = int(max_wave_rays)
         self.h = C.c_void_p()
         check(ctx._l.sg_server_create(ctx.h, C.byref(cfg), C.byref(self.h)), "sg_server_create")
+        ctx._own(self)
         self._keep: dict[int, tuple] = {}
         self._grids: list = []
 

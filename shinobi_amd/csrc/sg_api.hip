@@ -113,6 +113,7 @@ static TriSoA scene_soa(const sg_scene* s) {
 }
 
 static sg_status scene_upload(sg_scene* s, const float* const soa9[9], int on_dev, hipStream_t st) {
+    clear_last_error();
     const size_t b = (size_t)s->n * sizeof(float);
     for (int i = 0; i < 9; ++i) {
         if (!soa9[i]) return SG_ERR_INVALID_ARG;
@@ -131,6 +132,7 @@ sg_status sg_scene_create(sg_ctx* ctx, const float* const soa9[9], uint32_t n, i
     HIPCHK(hipSetDevice(ctx->device));
     sg_scene* s = new sg_scene();
     s->ctx = ctx;
+    s->device = ctx->device;
     s->n = n;
     s->stride = ((size_t)n + 63) & ~(size_t)63;
     if (hipMalloc(&s->soa, (size_t)9 * s->stride * sizeof(float)) != hipSuccess ||
@@ -158,7 +160,7 @@ sg_status sg_scene_update(sg_scene* s, const float* const soa9[9], int src_on_de
 
 sg_status sg_scene_destroy(sg_scene* s) {
     if (!s) return SG_ERR_INVALID_ARG;
-    hipSetDevice(s->ctx->device);
+    hipSetDevice(s->device);
     hipFree(s->soa);
     hipFree(s->rec);
     delete s;
@@ -328,9 +330,11 @@ sg_status sg_build(sg_ctx* ctx, sg_scene* scene, const sg_build_params* p, void*
     if (p->trace_block != 0 && p->trace_block != 64 && p->trace_block != 128 && p->trace_block != 256)
         return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
+    clear_last_error();
     hipStream_t st = (hipStream_t)stream;
     sg_grid* g = new sg_grid();
     g->ctx = ctx;
+    g->device = ctx->device;
     g->scene = scene;
     g->p = *p;
     g->kind = p->kind;
@@ -431,6 +435,7 @@ static sg_status rebuild_impl(sg_ctx* ctx, sg_scene* scene, sg_grid* g, void* st
     if (!ctx || !scene || !g) return SG_ERR_INVALID_ARG;
     if (scene->n != g->n_tris) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
+    clear_last_error();
     g->scene = scene;
     g->stream = (hipStream_t)stream;
     enqueue_build(g, fixed, g->stream, marks, n_marks);
@@ -527,7 +532,7 @@ sg_status sg_grid_download(sg_grid* g, int which, void* host, size_t bytes) {
 
 sg_status sg_grid_destroy(sg_grid* g) {
     if (!g) return SG_ERR_INVALID_ARG;
-    hipSetDevice(g->ctx->device);
+    hipSetDevice(g->device);
     if (g->pool) {
         hipStreamSynchronize(g->stream);
         hipFree(g->pool);
@@ -544,6 +549,7 @@ static sg_status trace_common(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint
         if (!rays->o[i] || !rays->d[i]) return SG_ERR_INVALID_ARG;
     if (!rays->tmax) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
+    clear_last_error();
     launch_trace(grid_view(g), ray_soa(rays), n, d_count, d_prim, d_t, d_stats, shadow, g->p.trace_block,
                  (hipStream_t)stream);
     HIPCHK(hipGetLastError());
@@ -565,6 +571,7 @@ sg_status sg_bounce(sg_ctx* ctx, sg_scene* scene, const sg_rays* rays, const uin
                     uint32_t* d_count, void* stream) {
     if (!ctx || !scene || !rays || !d_prim || !d_t || !out || !d_count) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
+    clear_last_error();
     void* scratch = nullptr;
     sg_status r = ctx_scratch(ctx, bounce_scratch_bytes(n), &scratch);
     if (r != SG_OK) return r;
@@ -584,6 +591,7 @@ sg_status sg_bounce(sg_ctx* ctx, sg_scene* scene, const sg_rays* rays, const uin
 sg_status sg_scan_u32(sg_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint32_t n, int inclusive, void* stream) {
     if (!ctx || (!d_in && n) || (!d_out && n)) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
+    clear_last_error();
     void* scratch = nullptr;
     sg_status r = ctx_scratch(ctx, scan_scratch_bytes(n), &scratch);
     if (r != SG_OK) return r;
@@ -604,6 +612,7 @@ sg_status sg_sort_pairs(sg_ctx* ctx, uint32_t* d_keys, uint32_t* d_vals, uint32_
     if (n >= (1u << 30)) return SG_ERR_INVALID_ARG;
     if (n < 2) return SG_OK;
     HIPCHK(hipSetDevice(ctx->device));
+    clear_last_error();
     hipStream_t st = (hipStream_t)stream;
     // scratch: [64 words: n][keys n][vals n][sort scratch]
     const size_t sw = sort_scratch_words(n, key_bits, radix_bits);

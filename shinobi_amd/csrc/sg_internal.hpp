@@ -19,6 +19,7 @@ struct sg_ctx {
 
 struct sg_scene {
     sg_ctx* ctx;
+    int device;               // the context's device, kept so destroy never reads the context
     uint32_t n;
     size_t stride = 0;        // floats between the SoA arrays (multiple of 64: 256-byte aligned arrays)
     float* soa = nullptr;     // 9*stride floats
@@ -27,6 +28,7 @@ struct sg_scene {
 
 struct sg_grid {
     sg_ctx* ctx;
+    int device;               // as sg_scene::device
     sg_scene* scene;
     sg_build_params p;
     int kind;
@@ -56,6 +58,11 @@ struct sg_grid {
         hipError_t e_ = (x);                        \
         if (e_ != hipSuccess) return hip_status(e_); \
     } while (0)
+
+// hipGetLastError reports the last error of any earlier HIP call of this
+// thread; an entry point that checks its own launches with it first drops
+// what earlier, unrelated calls left behind
+inline void clear_last_error() { (void)hipGetLastError(); }
 
 inline sg_status hip_status(hipError_t e) {
     if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return SG_ERR_OUT_OF_MEMORY;

@@ -122,6 +122,7 @@ sg_status sg_batch_commit(sg_batch* b, void* stream) {
 sg_status sg_batch_launch(sg_batch* b, int shadow, void* stream) {
     if (!b) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(b->ctx->device));
+    clear_last_error();
     launch_trace_items(b->dev, b->dev_items, b->dev_chunks, b->counter, shadow != 0, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return SG_OK;
@@ -397,6 +398,7 @@ namespace {
 // the results straight back; the copies are DMA transfers when the callers'
 // memory is pinned (sg_host_alloc)
 sg_status launch_wave(sg_server* s, Slot& sl) {
+    clear_last_error();
     const bool plain = s->cfg.mode == SG_SERVE_PLAIN;
     char* dbase = s->dev + sl.off;
     // device layout: [rays of every request][prim, t of every request]

@@ -66,6 +66,8 @@ SG_D void mt_test(const float4 q0, const float4 q1, const float q2, uint32_t it,
 }
 
 constexpr uint32_t kCoopMin = 64;   // references from which the wave shares a cell's tests
+constexpr int kTopBatch = 16;       // lanes waiting for a top step that make the wave take it
+constexpr uint32_t kTopWaitMax = 3; // iterations a lane waits for it at most
 
 // The wave tests the references [b, e) of lane L's cell against lane L's ray:
 // every active lane takes a strided share and the wave keeps the minimum
@@ -366,9 +368,16 @@ SG_D void walk_twolevel(const GridView& g, const GParams& p, Ray& r, Walk& w, ui
         }
     };
     if (w.active) top_cell();
+    // A lane done with its top cell waits (top_wait) until the wave takes its
+    // top steps together: at least kTopBatch lanes waiting, a lane waiting
+    // kTopWaitMax iterations, or no lane with anything else to do.  The top
+    // step and the set-up of the next leaf walk are the longest code of the
+    // loop; batching them runs that code in fewer iterations.
+    bool top_wait = false;
+    uint32_t waited = 0;
     while (__ballot(w.active)) {
         refs_step(g, q, r);
-        if (w.active && q.it >= q.e) {
+        if (w.active && !top_wait && q.it >= q.e) {
             bool top_next = !in_leaf;
             if (in_leaf) {
                 // a leaf cell is done: t check, leaf DDA step, next leaf cell
@@ -384,16 +393,28 @@ SG_D void walk_twolevel(const GridView& g, const GParams& p, Ray& r, Walk& w, ui
                     top_next = true;
                 }
             }
-            if (top_next) {
-                // the top cell is done: t check, top DDA step, next top cell
-                if (SHADOW ? r.t < kShadowT : !(r.t > w.tMax.x || r.t > w.tMax.y || r.t > w.tMax.z)) {
-                    w.active = false;
+            top_wait = top_next;
+        }
+        const uint64_t waiting = __ballot(w.active && top_wait);
+        if (waiting) {
+            const bool go = __popcll(waiting) >= kTopBatch || !__ballot(w.active && !top_wait) ||
+                            __ballot(w.active && top_wait && waited >= kTopWaitMax);
+            if (w.active && top_wait) {
+                if (!go) {
+                    ++waited;
                 } else {
-                    ++tsteps;
-                    const Step a = min_axis(w.tMax);
-                    w.tEntry = sel(a, w.tMax.x, w.tMax.y, w.tMax.z);
-                    w.active = dda_step(a, w.cell, w.tMax, r.rcp, p.cs, p.resf);
-                    if (w.active) top_cell();
+                    top_wait = false;
+                    waited = 0;
+                    // the top cell is done: t check, top DDA step, next top cell
+                    if (SHADOW ? r.t < kShadowT : !(r.t > w.tMax.x || r.t > w.tMax.y || r.t > w.tMax.z)) {
+                        w.active = false;
+                    } else {
+                        ++tsteps;
+                        const Step a = min_axis(w.tMax);
+                        w.tEntry = sel(a, w.tMax.x, w.tMax.y, w.tMax.z);
+                        w.active = dda_step(a, w.cell, w.tMax, r.rcp, p.cs, p.resf);
+                        if (w.active) top_cell();
+                    }
                 }
             }
         }
