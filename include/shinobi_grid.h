@@ -181,8 +181,11 @@ sg_status sg_trace_shadow(sg_ctx *ctx, sg_grid *grid, const sg_rays *rays, uint3
                           const uint32_t *d_count, uint32_t *d_prim, float *d_t, uint32_t *d_stats,
                           void *stream);
 /* one cosine-weighted diffuse bounce per hit (SimplePathTracer.hpp:115-166),
- * compacted in input order; *d_count = number of bounce rays; d_src (opt.)
- * = input ray index of each bounce ray.  n = capacity of the input; d_n
+ * compacted and grouped by direction class (4 bins of each component of the
+ * new direction, class = bx | by<<2 | bz<<4; input order within a class), the
+ * order in which incoherent secondary rays trace fastest; *d_count = number
+ * of bounce rays; d_src (opt.) = input ray index of each bounce ray.
+ * n = capacity of the input; d_n
  * (optional device pointer) = the number of input rays (<= n), so bounces
  * chain path segments without reading counts back to the host. */
 sg_status sg_bounce(sg_ctx *ctx, sg_scene *scene, const sg_rays *rays, const uint32_t *d_prim,

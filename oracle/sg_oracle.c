@@ -1072,9 +1072,21 @@ void or_brute_f64(const float *const v[9], uint32_t n, const float *const rays[7
 /* src/Integrator/SimplePathTracer.hpp:115-166 (one bounce per hit, no
  * Russian roulette) with the cosine sampler of
  * src/Utils/HemisphereSamplers.hpp:54-72 */
+/* direction class of a bounce ray: bin floor(2 (c + 1)) of each component,
+ * clamped to 0..3; class = bx | by << 2 | bz << 4 */
+static uint32_t dir_bin(float c)
+{
+    const float f = (c + 1.f) * 2.f;
+    return f < 1.f ? 0u : (f < 2.f ? 1u : (f < 3.f ? 2u : 3u));
+}
+
 uint32_t or_bounce(const float *const v[9], const float *const rays[7], const uint32_t *prim,
                    const float *t, uint32_t nrays, uint64_t seed, float *const out[7], uint32_t *src)
 {
+    /* the bounce rays in input order, then grouped stably by direction class */
+    float *tmp = (float *)malloc((size_t)7 * (nrays ? nrays : 1) * sizeof(float));
+    uint32_t *tsrc = (uint32_t *)malloc((size_t)(nrays ? nrays : 1) * sizeof(uint32_t));
+    uint32_t *cls = (uint32_t *)malloc((size_t)(nrays ? nrays : 1) * sizeof(uint32_t));
     uint32_t m = 0;
     for (uint32_t i = 0; i < nrays; ++i) {
         if (prim[i] == OR_MISS) continue;
@@ -1094,11 +1106,24 @@ uint32_t or_bounce(const float *const v[9], const float *const rays[7], const ui
         float rr = sqrtf(1.f - u2);
         v3 rd = mk(sn * rr, cn * rr, sqrtf(u2));
         v3 nd = add(add(scl(ta, rd.x), scl(tb, rd.y)), scl(n, rd.z));
-        out[0][m] = no.x; out[1][m] = no.y; out[2][m] = no.z;
-        out[3][m] = nd.x; out[4][m] = nd.y; out[5][m] = nd.z;
-        out[6][m] = FLT_MAX;
-        src[m] = i;
+        float *r = tmp + (size_t)7 * m;
+        r[0] = no.x; r[1] = no.y; r[2] = no.z;
+        r[3] = nd.x; r[4] = nd.y; r[5] = nd.z;
+        r[6] = FLT_MAX;
+        tsrc[m] = i;
+        cls[m] = dir_bin(nd.x) | dir_bin(nd.y) << 2 | dir_bin(nd.z) << 4;
         ++m;
     }
+    uint32_t start[65] = {0};
+    for (uint32_t k = 0; k < m; ++k) ++start[cls[k] + 1];
+    for (int c = 0; c < 64; ++c) start[c + 1] += start[c];
+    for (uint32_t k = 0; k < m; ++k) {
+        const uint32_t p = start[cls[k]]++;
+        for (int a = 0; a < 7; ++a) out[a][p] = tmp[(size_t)7 * k + a];
+        src[p] = tsrc[k];
+    }
+    free(tmp);
+    free(tsrc);
+    free(cls);
     return m;
 }

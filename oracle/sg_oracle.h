@@ -128,8 +128,10 @@ void or_brute_f64(const float *const v[9], uint32_t n,
 
 /* ---- secondary rays --------------------------------------------------------
  * One cosine-weighted diffuse bounce per hit (SimplePathTracer.hpp:115-166),
- * compacted in primary order.  Returns the number of rays written; src[i] is
- * the primary ray each bounce came from. */
+ * compacted and grouped stably by direction class (bin floor(2 (c + 1)) of
+ * each component of the new direction, clamped to 0..3; class = bx | by<<2 |
+ * bz<<4), the order the library's generator writes.  Returns the number of
+ * rays written; src[i] is the primary ray each bounce came from. */
 uint32_t or_bounce(const float *const v[9], const float *const rays[7],
                    const uint32_t *prim, const float *t, uint32_t nrays,
                    uint64_t seed, float *const out[7], uint32_t *src);

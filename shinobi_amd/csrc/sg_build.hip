@@ -749,16 +749,6 @@ __global__ __launch_bounds__(1024) void k_leaf_scan(GridHeader* h, BuildArgs a, 
 // a wave come from ballots over the digit bits (computed once for the digit
 // histogram and again for the scatter, which keeps registers low); waves are
 // ordered through per-wave digit counts, rounds through running bases.
-SG_D uint64_t digit_peers(bool ok, uint32_t d, int db) {
-    uint64_t peers = __ballot(ok);
-    for (int b = 0; b < db; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t m = __ballot(bit);
-        peers &= bit ? m : ~m;
-    }
-    return peers;
-}
-
 SG_D void lds_sort_pass(const uint32_t* in, uint32_t* out, uint32_t n, int shift, int db, uint32_t* s_base,
                         uint32_t* s_wc, uint32_t* s_scan) {
     constexpr int NW = kLeafBlock / 64;

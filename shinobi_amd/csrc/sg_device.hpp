@@ -76,6 +76,18 @@ SG_D uint32_t block_prefix_of(const uint32_t* __restrict__ sums, uint32_t b, uin
     return block_sum<NW>(acc, lds);
 }
 
+// lanes of the wave with ok set and the same low db bits of d (a wave64
+// "match"): one ballot per bit
+SG_D uint64_t digit_peers(bool ok, uint32_t d, int db) {
+    uint64_t peers = __ballot(ok);
+    for (int b = 0; b < db; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
 // ------------------------------------------- correctly rounded 1/x and a/b
 // Both return exactly what IEEE division returns, in fewer instructions
 // than the compiler's general division sequence (div_scale, rcp, five

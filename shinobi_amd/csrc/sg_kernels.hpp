@@ -129,8 +129,9 @@ void init_trace_items();
 // (block: threads per block, 64, 128 or 256)
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim,
                   float* t, uint32_t* stats, bool shadow, int block, hipStream_t st);
-// one cosine-weighted bounce per hit, compacted in primary order; *d_count
-// receives the number of bounce rays.  scratch: bounce_scratch_bytes(n).
+// one cosine-weighted bounce per hit, compacted and grouped by direction
+// class (input order within a class); *d_count receives the number of
+// bounce rays.  scratch: bounce_scratch_bytes(n).
 size_t bounce_scratch_bytes(uint32_t n);
 void launch_bounce(const TriSoA& tris, const RaySoA& r, const uint32_t* prim, const float* t, uint32_t n,
                    const uint32_t* d_n, uint64_t seed, const RaySoAOut& out, uint32_t* src, uint32_t* d_count,

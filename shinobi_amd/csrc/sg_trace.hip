@@ -564,57 +564,20 @@ void launch_trace_items(const TraceItem* items, uint32_t n_items, uint32_t total
 // without Russian roulette): frame from the hit triangle's normalised edge1
 // and n x edge1, n flipped against the ray, origin offset 0.001 n, cosine
 // sample (src/Utils/HemisphereSamplers.hpp:54-72) from a counter hash of the
-// primary ray id.  Compacted in primary order: per-tile hit counts, a
-// one-block scan of the counts, then each tile writes at its offset.
+// primary ray id.
+//
+// The bounce rays are compacted and grouped by direction class: 4 bins per
+// component of the new direction, 64 classes, and within a class in input
+// order.  Incoherent secondary rays traced in this order share more cells
+// and triangles per wave (heightfield bounce trace 1.40 -> 1.24 ms); the
+// grouping is a counting sort fused into the compaction: per-tile class
+// counts, a scan of each class over the tiles, then each tile writes its
+// rays at their class offsets with wave-ballot ranks (stable).
 constexpr int kBounceTile = kBlock;
+constexpr uint32_t kDirClasses = 64;
 
-__global__ __launch_bounds__(kBlock) void k_bounce_count(const uint32_t* __restrict__ prim, uint32_t cap,
-                                                         const uint32_t* d_n, uint32_t* __restrict__ sums) {
-    __shared__ uint32_t s_red[kBlock / 64];
-    const uint32_t n = d_n ? min(*d_n, cap) : cap;
-    const uint32_t i = blockIdx.x * kBounceTile + threadIdx.x;
-    const bool hit = i < n && prim[i] != kMiss;
-    const uint32_t s = block_sum<kBlock / 64>(hit ? 1u : 0u, s_red);
-    if (threadIdx.x == 0) sums[blockIdx.x] = s;
-}
-
-// exclusive scan of the tile counts in place; the total goes to *d_count
-__global__ __launch_bounds__(1024) void k_bounce_scan(uint32_t* sums, uint32_t tiles, uint32_t* d_count) {
-    __shared__ uint32_t s_scan[16];
-    uint32_t run = 0;
-    for (uint32_t c0 = 0; c0 < tiles; c0 += 1024 * 4) {
-        const uint32_t t0 = c0 + threadIdx.x * 4;
-        uint32_t v[4], sum = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            v[k] = t0 + k < tiles ? sums[t0 + k] : 0u;
-            sum += v[k];
-        }
-        uint32_t tot;
-        uint32_t pre = run + block_excl_scan<16>(sum, &tot, s_scan);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (t0 + k < tiles) sums[t0 + k] = pre;
-            pre += v[k];
-        }
-        run += tot;
-    }
-    if (threadIdx.x == 0) *d_count = run;
-}
-
-__global__ __launch_bounds__(kBlock) void k_bounce(TriSoA tris, RaySoA rays, const uint32_t* __restrict__ prim,
-                                                   const float* __restrict__ tin, uint32_t cap, const uint32_t* d_n,
-                                                   uint64_t seed, RaySoAOut out, uint32_t* __restrict__ src,
-                                                   const uint32_t* __restrict__ sums) {
-    __shared__ uint32_t s_scan[kBlock / 64];
-    const uint32_t n = d_n ? min(*d_n, cap) : cap;
-    const uint32_t i = blockIdx.x * kBounceTile + threadIdx.x;
-    const bool hit = i < n && prim[i] != kMiss;
-    uint32_t agg;
-    const uint32_t local = block_excl_scan<kBlock / 64>(hit ? 1u : 0u, &agg, s_scan);
-    if (!hit) return;
-    const uint32_t m = sums[blockIdx.x] + local;
-    const uint32_t tri = prim[i];
+SG_D void bounce_ray(const TriSoA& tris, const RaySoA& rays, uint32_t i, uint32_t tri, float t, uint64_t seed,
+                     v3& no, v3& nd) {
     const v3 o = mk(rays.a[0][i], rays.a[1][i], rays.a[2][i]);
     const v3 d = mk(rays.a[3][i], rays.a[4][i], rays.a[5][i]);
     const float4* rq = tris.rec + 4 * (size_t)tri;
@@ -624,13 +587,101 @@ __global__ __launch_bounds__(kBlock) void k_bounce(TriSoA tris, RaySoA rays, con
     ta = normalize(ta);
     tb = cross(nn, ta);
     if (dot(nn, d) > 0.f) nn = mk(-nn.x, -nn.y, -nn.z);
-    const v3 no = (o + scl(d, tin[i])) + scl(nn, 0.001f);
+    no = (o + scl(d, t)) + scl(nn, 0.001f);
     const float u1 = u01(hash3(seed, i, 0)), u2 = u01(hash3(seed, i, 1));
     float sn, cn;
     sincos_2pi(u1, &sn, &cn);
     const float rr = sqrtf(1.f - u2);
     const v3 rd = mk(sn * rr, cn * rr, sqrtf(u2));
-    const v3 nd = (scl(ta, rd.x) + scl(tb, rd.y)) + scl(nn, rd.z);
+    nd = (scl(ta, rd.x) + scl(tb, rd.y)) + scl(nn, rd.z);
+}
+
+// direction class: bin floor(2 (c + 1)) of each component, clamped to 0..3
+SG_HD uint32_t dir_class(v3 d) {
+    auto bin = [](float c) -> uint32_t {
+        const float f = (c + 1.f) * 2.f;
+        return f < 1.f ? 0u : (f < 2.f ? 1u : (f < 3.f ? 2u : 3u));
+    };
+    return bin(d.x) | bin(d.y) << 2 | bin(d.z) << 4;
+}
+
+// counts[c * tiles + tile] = bounce rays of class c in the tile
+__global__ __launch_bounds__(kBlock) void k_bounce_count(TriSoA tris, RaySoA rays, const uint32_t* __restrict__ prim,
+                                                         const float* __restrict__ tin, uint32_t cap,
+                                                         const uint32_t* d_n, uint64_t seed,
+                                                         uint32_t* __restrict__ counts, uint32_t tiles) {
+    __shared__ uint32_t s_c[kDirClasses];
+    const uint32_t n = d_n ? min(*d_n, cap) : cap;
+    const uint32_t i = blockIdx.x * kBounceTile + threadIdx.x;
+    if (threadIdx.x < kDirClasses) s_c[threadIdx.x] = 0;
+    __syncthreads();
+    if (i < n && prim[i] != kMiss) {
+        v3 no, nd;
+        bounce_ray(tris, rays, i, prim[i], tin[i], seed, no, nd);
+        atomicAdd(&s_c[dir_class(nd)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kDirClasses) counts[(size_t)threadIdx.x * tiles + blockIdx.x] = s_c[threadIdx.x];
+}
+
+// block c: exclusive scan of class c's tile counts in place, its total to
+// totals[c]
+__global__ __launch_bounds__(1024) void k_bounce_scan(uint32_t* counts, uint32_t tiles, uint32_t* totals) {
+    __shared__ uint32_t s_scan[16];
+    uint32_t* row = counts + (size_t)blockIdx.x * tiles;
+    uint32_t run = 0;
+    for (uint32_t c0 = 0; c0 < tiles; c0 += 1024 * 4) {
+        const uint32_t t0 = c0 + threadIdx.x * 4;
+        uint32_t v[4], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[k] = t0 + k < tiles ? row[t0 + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t tot;
+        uint32_t pre = run + block_excl_scan<16>(sum, &tot, s_scan);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (t0 + k < tiles) row[t0 + k] = pre;
+            pre += v[k];
+        }
+        run += tot;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = run;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bounce(TriSoA tris, RaySoA rays, const uint32_t* __restrict__ prim,
+                                                   const float* __restrict__ tin, uint32_t cap, const uint32_t* d_n,
+                                                   uint64_t seed, RaySoAOut out, uint32_t* __restrict__ src,
+                                                   const uint32_t* __restrict__ counts,
+                                                   const uint32_t* __restrict__ totals, uint32_t tiles,
+                                                   uint32_t* d_count) {
+    constexpr int NW = kBlock / 64;
+    __shared__ uint32_t s_base[kDirClasses], s_wc[NW][kDirClasses];
+    const uint32_t n = d_n ? min(*d_n, cap) : cap;
+    const uint32_t i = blockIdx.x * kBounceTile + threadIdx.x;
+    if (threadIdx.x < kDirClasses) {
+        uint32_t base = 0;
+        for (uint32_t c = 0; c < threadIdx.x; ++c) base += totals[c];
+        s_base[threadIdx.x] = base + counts[(size_t)threadIdx.x * tiles + blockIdx.x];
+        if (blockIdx.x == 0 && threadIdx.x == kDirClasses - 1) *d_count = base + totals[kDirClasses - 1];
+    }
+    for (uint32_t k = threadIdx.x; k < NW * kDirClasses; k += kBlock) (&s_wc[0][0])[k] = 0;
+    const bool hit = i < n && prim[i] != kMiss;
+    v3 no = mk(0.f, 0.f, 0.f), nd = no;
+    uint32_t c = 0;
+    if (hit) {
+        bounce_ray(tris, rays, i, prim[i], tin[i], seed, no, nd);
+        c = dir_class(nd);
+    }
+    const uint64_t peers = digit_peers(hit, c, 6);
+    const uint32_t rank = (uint32_t)__popcll(peers & lanemask_lt());
+    __syncthreads();
+    if (hit && rank == 0) s_wc[wave_id()][c] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (!hit) return;
+    uint32_t m = s_base[c] + rank;
+    for (int w = 0; w < wave_id(); ++w) m += s_wc[w][c];
     out.a[0][m] = no.x;
     out.a[1][m] = no.y;
     out.a[2][m] = no.z;
@@ -647,7 +698,7 @@ __global__ void k_set_u32(uint32_t* p, uint32_t v) {
 
 size_t bounce_scratch_bytes(uint32_t n) {
     const size_t tiles = ((size_t)n + kBounceTile - 1) / kBounceTile;
-    return ((tiles + 4) * sizeof(uint32_t) + 15) & ~(size_t)15;
+    return ((kDirClasses * (tiles + 1) + 4) * sizeof(uint32_t) + 15) & ~(size_t)15;
 }
 
 void launch_bounce(const TriSoA& tris, const RaySoA& r, const uint32_t* prim, const float* t, uint32_t n,
@@ -658,10 +709,13 @@ void launch_bounce(const TriSoA& tris, const RaySoA& r, const uint32_t* prim, co
         return;
     }
     const uint32_t tiles = (n + kBounceTile - 1) / kBounceTile;
-    uint32_t* sums = reinterpret_cast<uint32_t*>(scratch);
-    hipLaunchKernelGGL(k_bounce_count, dim3(tiles), dim3(kBlock), 0, st, prim, n, d_n, sums);
-    hipLaunchKernelGGL(k_bounce_scan, dim3(1), dim3(1024), 0, st, sums, tiles, d_count);
-    hipLaunchKernelGGL(k_bounce, dim3(tiles), dim3(kBlock), 0, st, tris, r, prim, t, n, d_n, seed, out, src, sums);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(scratch);
+    uint32_t* totals = counts + (size_t)kDirClasses * tiles;
+    hipLaunchKernelGGL(k_bounce_count, dim3(tiles), dim3(kBlock), 0, st, tris, r, prim, t, n, d_n, seed, counts,
+                       tiles);
+    hipLaunchKernelGGL(k_bounce_scan, dim3(kDirClasses), dim3(1024), 0, st, counts, tiles, totals);
+    hipLaunchKernelGGL(k_bounce, dim3(tiles), dim3(kBlock), 0, st, tris, r, prim, t, n, d_n, seed, out, src, counts,
+                       totals, tiles, d_count);
 }
 
 }  // namespace sg

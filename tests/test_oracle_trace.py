@@ -106,14 +106,30 @@ def test_non_finite_rays_terminate(oracle, scene):
         assert prim.shape == (6,)
 
 
+def _dir_class(d):
+    """4 bins per component of the direction, class = bx | by<<2 | bz<<4:
+    bin floor(2 (c + 1)) clamped to 0..3, in binary32 as the generator
+    computes it (-1e-9 + 1 rounds to 1, so tiny negatives share bin 2)"""
+    f = (d.astype(np.float32) + np.float32(1)) * np.float32(2)
+    bins = np.digitize(f, np.array([1, 2, 3], np.float32))
+    return bins[0] | bins[1] << 2 | bins[2] << 4
+
+
 def test_bounce_rays(oracle, scene):
-    """one cosine-weighted bounce per hit: compacted in primary order, unit
-    directions in the hemisphere facing away from the incoming ray, origin
-    0.001 off the surface"""
+    """one cosine-weighted bounce per hit: every hit exactly once, grouped by
+    direction class and in primary order within a class, unit directions in
+    the hemisphere facing away from the incoming ray, origin 0.001 off the
+    surface"""
     r = sa.gen_random_rays(40, 5000, 0.0, 1.0)
     g = oracle.build_uniform(scene, top_divisor=1)
     prim, t = g.trace(r)
     b, src = oracle.bounce(scene, r, prim, t, 7)
+    cls = _dir_class(b[3:6])
+    assert (np.diff(cls) >= 0).all()
+    assert all((np.diff(src[cls == c].astype(np.int64)) > 0).all() for c in np.unique(cls))
+    assert len(np.unique(cls)) > 16
+    order = np.argsort(src)
+    b, src = b[:, order], src[order]
     hit = np.flatnonzero(prim != MISS)
     assert np.array_equal(src, hit)
     d = b[3:6].astype(np.float64)
