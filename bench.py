@@ -7,9 +7,11 @@ Workload (one step = one pass of the hot path over one batch):
   step   1. two-level grid rebuild from the SoA triangles (bounds,
             resolution-fixed top level lambda 5 / 6, leaf lambda 2, every
             phase on the device, no host sync)
-         2. closest-hit trace of 1920x1080 primary rays from (0.5,-0.3,0.6)
+         2. closest-hit trace of 1920x1080 primary rays from (0.5,-0.3,0.6),
+            ordered in 8x8 pixel tiles (the row-major time is reported as
+            trace_primary_rowmajor_ms)
          3. one cosine-weighted diffuse bounce per hit (device generator,
-            compacted)
+            compacted and grouped by direction class)
          4. closest-hit trace of the bounce rays (count read on the device)
 value  = closest-hit Mrays/s = (primary + bounce rays) / (time of the two
          trace kernels), measured with HIP events on the stream the kernels
@@ -33,6 +35,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 FP32_PEAK_TFLOPS = 157.3       # MI355X vector FP32, TFLOP/s
+
+PRIMARY_TILE = 8   # primary rays ordered in 8x8 pixel tiles (workloads.tile_order)
 
 PHASES = ["bounds", "top_emit", "top_sort", "top_ranges", "leaf_cells", "leaf_emit", "leaf_sort", "leaf_ranges"]
 
@@ -372,7 +376,7 @@ def main() -> int:
     import shinobi_amd as sa
     from shinobi_amd import workloads as W
 
-    w = W.heightfield2m()
+    w = W.heightfield2m(tile=PRIMARY_TILE)
     n_tris = w.scene.shape[1]
     n_rays = w.rays.shape[1]
     ctx = sa.Context(0)
@@ -439,6 +443,18 @@ def main() -> int:
         tp_ms.append(e[8].elapsed_ms(e[9]))
         bn_ms.append(e[9].elapsed_ms(e[10]))
         tb_ms.append(e[10].elapsed_ms(e[11]))
+    # the same primary rays in row-major pixel order, for reference
+    rays_rm = sa.RayBatch(ctx, W.heightfield_camera(2, 1920, 1080))
+    e_a, e_b = ctx.event(), ctx.event()
+    rm = []
+    for _ in range(6):
+        e_a.record()
+        grid.trace(rays_rm, prim=prim, t=tbuf)
+        e_b.record()
+        ctx.sync()
+        rm.append(e_a.elapsed_ms(e_b))
+    rowmajor_ms = float(np.median(rm[1:]))
+    del rays_rm
     info2 = grid.info()
     if info2.status != 0 or (info2.P1, info2.P2, info2.C2) != (info.P1, info.P2, info.C2):
         raise RuntimeError("rebuild changed the grid")
@@ -496,11 +512,13 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "heightfield2m: 2,093,058-tri height field, two-level grid "
                                "(lambda 5 / 6 top, lambda 2 leaf), 1920x1080 primary + 1 diffuse bounce per hit",
+                   "primary_order": f"{PRIMARY_TILE}x{PRIMARY_TILE} pixel tiles",
                    "n_tris": n_tris, "primary_rays": n_rays, "bounce_rays": m, "res_top": list(info.res),
                    "C1": info.C1, "P1": info.P1, "C2": info.C2, "P2": info.P2},
         "build_ms": round(mean(build_ms), 4),
         "build_ms_median": round(float(np.median(build_ms)), 4),
         "trace_primary_ms": round(mean(tp_ms), 4),
+        "trace_primary_rowmajor_ms": round(rowmajor_ms, 4),
         "bounce_gen_ms": round(mean(bn_ms), 4),
         "trace_bounce_ms": round(mean(tb_ms), 4),
         "wall_s": round(t_wall, 3),

@@ -6,9 +6,9 @@ parameters), shared by the tests, the smoke check and bench.py.
   60 degree vertical FOV; uniform grid, lambda 5.
 * ``heightfield2m`` -- 1024x1024-vertex height field over [0,1]^2 =
   2,093,058 triangles, 8-octave value noise of amplitude 0.2 (seed 2);
-  1920x1080 rays from (0.5,-0.3,0.6) toward the field centre plus one
-  cosine-weighted bounce per hit (seed 102); two-level grid, lambda 5 / 6 top,
-  lambda 2 leaf, 16-reference leaf threshold.
+  1920x1080 rays from (0.5,-0.3,0.6) toward the field centre (row-major, or
+  tile by tile) plus one cosine-weighted bounce per hit (seed 102); two-level
+  grid, lambda 5 / 6 top, lambda 2 leaf, 16-reference leaf threshold.
 * ``cluster`` -- 500,000 tiny triangles in a sphere of radius 0.02 at
   (0.5,0.5,0.5) plus 2,000 large triangles in [0,10]^3 (seed 3); two-level.
 * ``scene_file`` -- an OBJ scene and its camera.cfg (the reference's ruins
@@ -56,9 +56,26 @@ def heightfield_camera(seed: int, res_x: int, res_y: int, amplitude: float = 0.2
     return gen_camera_rays(eye, fwd, (0.0, 0.0, 1.0), 60.0, res_x, res_y)
 
 
-def heightfield2m(grid: int = 1024, res_x: int = 1920, res_y: int = 1080, seed: int = 2) -> Workload:
+def tile_order(res_x: int, res_y: int, tile: int) -> np.ndarray:
+    """Row-major pixel ids visited tile by tile: tile x tile blocks in
+    row-major order, row-major inside a block (partial blocks at the right
+    and bottom edges keep their pixels).  tile = 1 is the identity."""
+    ids = np.arange(res_x * res_y, dtype=np.int64).reshape(res_y, res_x)
+    if tile <= 1:
+        return ids.reshape(-1)
+    out = [ids[y0:y0 + tile, x0:x0 + tile].reshape(-1)
+           for y0 in range(0, res_y, tile) for x0 in range(0, res_x, tile)]
+    return np.concatenate(out)
+
+
+def heightfield2m(grid: int = 1024, res_x: int = 1920, res_y: int = 1080, seed: int = 2,
+                  tile: int = 1) -> Workload:
+    """tile > 1: the same primary rays ordered tile by tile (tile_order), the
+    pixel order a renderer uses to keep the rays of a wave close together"""
     scene = gen_heightfield(seed, grid, 0.2)
     rays = heightfield_camera(seed, res_x, res_y)
+    if tile > 1:
+        rays = np.ascontiguousarray(rays[:, tile_order(res_x, res_y, tile)])
     return Workload("heightfield2m", SG_GRID_TWO_LEVEL, scene, rays,
                     dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16), bounce_seed=seed + 100,
                     oracle=dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16))
