@@ -975,7 +975,7 @@ void or_trace_shadow(const or_grid *g, const float *const v[9], const float *con
 }
 
 /* ------------------------------------------------------------------------- */
-/* brute-force judges                                                        */
+/* brute-force referees                                                      */
 /* ------------------------------------------------------------------------- */
 
 typedef struct {

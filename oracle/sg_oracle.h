@@ -121,7 +121,7 @@ void or_trace_shadow(const or_grid *g, const float *const v[9],
 void or_brute_f32(const float *const v[9], uint32_t n,
                   const float *const rays[7], uint32_t nrays,
                   uint32_t *prim, float *t, int nthreads);
-/* FP64 judge: MT in double with the original direction */
+/* FP64 referee: MT in double with the original direction */
 void or_brute_f64(const float *const v[9], uint32_t n,
                   const float *const rays[7], uint32_t nrays,
                   uint32_t *prim, double *t, int nthreads);

@@ -85,8 +85,8 @@ def test_twolevel_heightfield2m_primary_and_bounce(ctx, oracle):
     assert frac2 == 1.0
 
 
-def test_fp64_judge_small_scene(ctx, oracle):
-    """FP64 brute force over all triangles is the judge on a small scene."""
+def test_fp64_referee_small_scene(ctx, oracle):
+    """FP64 brute force over all triangles is the referee on a small scene."""
     soa = sa.gen_soup(11, 2000, 0.03)
     rays = sa.gen_random_rays(12, 4000, 0.0, 1.0)
     for kind in (0, 1):

@@ -2,7 +2,7 @@
 (src/Control/Algorithm/SimpleGridTraverser.h:274-401,
 src/Control/Algorithm/TwoLevelGridTraverser.h:445-671,
 src/Intersector/SimpleIntersector.hpp:50-89): FP64 brute force over every
-triangle is the judge, FP32 brute force with the same formula must find the
+triangle is the referee, FP32 brute force with the same formula must find the
 same hit wherever the grid visits the hit triangle, and the reference's
 IntersectorTest scene pins tie-breaking."""
 import numpy as np
@@ -24,7 +24,7 @@ def rays():
 
 
 @pytest.mark.parametrize("kind", [0, 1])
-def test_fp64_judge(oracle, scene, rays, kind):
+def test_fp64_referee(oracle, scene, rays, kind):
     g = (oracle.build_twolevel(scene, lambda_leaf=2.0) if kind else oracle.build_uniform(scene, top_divisor=1))
     prim, t = g.trace(rays)
     p64, t64 = oracle.brute_f64(scene, rays)
@@ -54,7 +54,7 @@ def test_tie_scene(oracle):
     """IntersectorTest (src/Test/IntersectorTest.h:179-258): triangle 0 in the
     plane z=-2, triangles 1..399 identical in the plane y=-4; camera at the
     origin looking along -y.  Strict '<' keeps the first of the equal-t
-    duplicates, so the answer is 0, 1 or a miss -- exactly the FP64 judge."""
+    duplicates, so the answer is 0, 1 or a miss -- exactly the FP64 referee."""
     n = 400
     soa = np.zeros((9, n), np.float32)
     soa[:, 0] = [-3, -10, -2, 3, -10, -2, 0, 10, -2]
@@ -179,7 +179,7 @@ def test_shadow_any_hit(oracle, scene, rays, kind):
     assert 0.05 < occ_any.mean() < 0.95
     hit = pa != MISS
     assert (ta[hit] >= tc[hit]).all()
-    # the reported triangle really is hit at the reported t (FP64 judge on that triangle alone)
+    # the reported triangle really is hit at the reported t (FP64 referee on that triangle alone)
     idx = np.flatnonzero(hit)[:400]
     for i in idx:
         tri = scene[:, pa[i]:pa[i] + 1]
