@@ -362,6 +362,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-scene measurement")
     ap.add_argument("--no-cluster", action="store_true", help="skip the dense-cluster path measurement")
     ap.add_argument("--no-ruins", action="store_true", help="skip the reference ruins scene measurement")
+    ap.add_argument("--no-rowmajor", action="store_true", help="skip the row-major primary trace measurement")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 64 Mi-element scan (k_scan_up reads 256 MiB, k_scan_down reads and writes "
                          "256 MiB): the reference kernels for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -444,17 +445,19 @@ def main() -> int:
         bn_ms.append(e[9].elapsed_ms(e[10]))
         tb_ms.append(e[10].elapsed_ms(e[11]))
     # the same primary rays in row-major pixel order, for reference
-    rays_rm = sa.RayBatch(ctx, W.heightfield_camera(2, 1920, 1080))
-    e_a, e_b = ctx.event(), ctx.event()
-    rm = []
-    for _ in range(6):
-        e_a.record()
-        grid.trace(rays_rm, prim=prim, t=tbuf)
-        e_b.record()
-        ctx.sync()
-        rm.append(e_a.elapsed_ms(e_b))
-    rowmajor_ms = float(np.median(rm[1:]))
-    del rays_rm
+    rowmajor_ms = None
+    if not args.no_rowmajor:
+        rays_rm = sa.RayBatch(ctx, W.heightfield_camera(2, 1920, 1080))
+        e_a, e_b = ctx.event(), ctx.event()
+        rm = []
+        for _ in range(6):
+            e_a.record()
+            grid.trace(rays_rm, prim=prim, t=tbuf)
+            e_b.record()
+            ctx.sync()
+            rm.append(e_a.elapsed_ms(e_b))
+        rowmajor_ms = round(float(np.median(rm[1:])), 4)
+        del rays_rm
     info2 = grid.info()
     if info2.status != 0 or (info2.P1, info2.P2, info2.C2) != (info.P1, info.P2, info.C2):
         raise RuntimeError("rebuild changed the grid")
@@ -518,7 +521,7 @@ def main() -> int:
         "build_ms": round(mean(build_ms), 4),
         "build_ms_median": round(float(np.median(build_ms)), 4),
         "trace_primary_ms": round(mean(tp_ms), 4),
-        "trace_primary_rowmajor_ms": round(rowmajor_ms, 4),
+        "trace_primary_rowmajor_ms": rowmajor_ms,
         "bounce_gen_ms": round(mean(bn_ms), 4),
         "trace_bounce_ms": round(mean(tb_ms), 4),
         "wall_s": round(t_wall, 3),

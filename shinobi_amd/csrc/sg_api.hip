@@ -203,6 +203,7 @@ static size_t grid_layout(sg_grid* g, char* pool) {
         }
         g->leaves = (uint32_t*)take((size_t)g->cap_C2 * 8 + 16);
     }
+    g->refs = (float4*)take((size_t)(two ? g->cap_P2 : g->cap_P1) * 48 + 16);
     if (g->p.keep_debug) {
         g->dbg_tri = (uint32_t*)take((size_t)N * 4 + 16);
         g->dbg_leafc = two ? (uint32_t*)take((size_t)g->cap_C1 * 4 + 16) : nullptr;
@@ -302,17 +303,19 @@ static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st, void* cons
     mark(3);
     const int f1 = g->passes1 & 1;
     launch_ranges(g->k1[f1], &g->hdr->P1, g->cap_P1, &g->hdr->C1, g->ranges1, &g->hdr->P1_valid, st);
+    if (!two) launch_ref_records(g->v1[f1], &g->hdr->P1_valid, nullptr, g->cap_P1, g->scene->rec, N, g->refs, st);
     mark(4);
     if (!two) return;
     launch_leaf_cells(g->hdr, a, g->ranges1, g->cells1, g->dbg_leafc, g->sums_cells, st);
     mark(5);
-    launch_leaf_emit(t, g->hdr, a, g->k1[f1], g->v1[f1], g->ranges1, g->cells1, g->k2[0], g->v2[0], g->dbg_pair,
-                     g->sums_leaf, g->leaf_info, st);
+    launch_leaf_emit(t, g->hdr, a, g->k1[f1], g->v1[f1], g->ranges1, g->cells1, g->k2[0], g->v2[0], g->refs,
+                     g->dbg_pair, g->sums_leaf, g->leaf_info, st);
     mark(6);
     launch_leaf_sort_fixup(g->k2[0], g->v2[0], g->k2[1], g->v2[1], g->hdr, g->cap_P2, g->key_bits2, g->rb, g->sort2,
                            st);
     mark(7);
     launch_ranges(g->k2[0], &g->hdr->P2, g->cap_P2, &g->hdr->C2, g->leaves, &g->hdr->P2_valid, st);
+    launch_ref_records(g->v2[0], &g->hdr->P2_valid, &g->hdr->leaf_sort, g->cap_P2, g->scene->rec, N, g->refs, st);
     mark(8);
 }
 

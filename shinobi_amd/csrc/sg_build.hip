@@ -837,6 +837,7 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* 
                                                          const uint32_t* __restrict__ top_vals,
                                                          const uint32_t* __restrict__ cells,
                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                         float4* __restrict__ refs,
                                                          const uint32_t* __restrict__ sums,
                                                          const uint32_t* __restrict__ info, uint32_t T) {
     constexpr int NW = kLeafBlock / 64;
@@ -897,6 +898,9 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* 
         if (idx < cap) {
             keys[idx] = key;
             vals[idx] = v;
+            // accepted pairs are at their final slots: their reference
+            // records too (the tile's triangles were read while staging)
+            if (j < acc_t) write_ref_record(refs, idx, t.rec, v, L.n_tris);
         }
     }
 }
@@ -1006,7 +1010,7 @@ void launch_leaf_cells(GridHeader* h, const BuildArgs& a, const uint32_t* ranges
 
 void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const uint32_t* top_keys,
                       const uint32_t* top_vals, const uint32_t* ranges, const uint32_t* cells, uint32_t* keys,
-                      uint32_t* vals, uint32_t* dbg, uint32_t* sums, uint32_t* info, hipStream_t st) {
+                      uint32_t* vals, float4* refs, uint32_t* dbg, uint32_t* sums, uint32_t* info, hipStream_t st) {
     const uint32_t T = (uint32_t)leaf_tiles(a.cap_P1);
     const uint32_t blocks = (a.cap_P1 + kTile - 1) / kTile;
     if (blocks)
@@ -1014,7 +1018,7 @@ void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const 
                            sums, info, T, dbg);
     hipLaunchKernelGGL(k_leaf_scan, dim3(1), dim3(1024), 0, st, h, a, sums, T);
     hipLaunchKernelGGL(k_leaf_emit, dim3(T), dim3(kLeafBlock), 0, st, t, h, a, top_keys, top_vals, cells, keys, vals,
-                       sums, info, T);
+                       refs, sums, info, T);
     hipLaunchKernelGGL(k_leaf_emit_large, dim3(T), dim3(kBlock), 0, st, t, h, a, top_keys, top_vals, cells, keys,
                        vals, sums, info, T);
 }
@@ -1027,6 +1031,27 @@ void launch_leaf_sort_fixup(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* 
         blocks = blocks < 1 ? 1 : (blocks > 2048 ? 2048 : blocks);
         hipLaunchKernelGGL(k_copy_pairs, dim3(blocks), dim3(kBlock), 0, st, k1, v1, k0, v0, &h->leaf_sort, cap);
     }
+}
+
+// d_gate (optional): nothing to do while *d_gate == 0 (two-level grids: the
+// LDS-sorted leaf tiles wrote their records in k_leaf_emit; only a global
+// leaf sort, GridHeader::leaf_sort > 0, leaves them to this pass)
+__global__ __launch_bounds__(kBlock) void k_ref_records(const uint32_t* __restrict__ indices, const uint32_t* d_valid,
+                                                        const uint32_t* d_gate, uint32_t cap,
+                                                        const float4* __restrict__ rec, uint32_t n_tris,
+                                                        float4* __restrict__ refs) {
+    if (d_gate && *d_gate == 0) return;
+    const uint32_t n = min(*d_valid, cap);
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        write_ref_record(refs, i, rec, indices[i], n_tris);
+}
+
+void launch_ref_records(const uint32_t* indices, const uint32_t* d_valid, const uint32_t* d_gate, uint32_t cap,
+                        const float4* rec, uint32_t n_tris, float4* refs, hipStream_t st) {
+    uint32_t blocks = (cap + kBlock * 4 - 1) / (kBlock * 4);
+    blocks = blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks);
+    hipLaunchKernelGGL(k_ref_records, dim3(blocks), dim3(kBlock), 0, st, indices, d_valid, d_gate, cap, rec, n_tris,
+                       refs);
 }
 
 // number of per-tile sums the emission kernels need for n items

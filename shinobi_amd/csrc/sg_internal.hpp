@@ -43,6 +43,7 @@ struct sg_grid {
     uint32_t *ranges1 = nullptr, *cells1 = nullptr;
     uint32_t *k2[2] = {nullptr, nullptr}, *v2[2] = {nullptr, nullptr};
     uint32_t* leaves = nullptr;
+    float4* refs = nullptr;   // per-reference records (launch_ref_records)
     uint32_t *dbg_tri = nullptr, *dbg_leafc = nullptr, *dbg_pair = nullptr;
     uint32_t *sums_top = nullptr, *sums_cells = nullptr, *sums_leaf = nullptr;   // per-tile sums
     uint32_t* leaf_info = nullptr;                                               // per leaf tile bounds
@@ -81,6 +82,7 @@ inline GridView grid_view(const sg_grid* g) {
     v.leaves = two ? g->leaves : nullptr;
     v.indices = two ? g->v2[f2] : g->v1[f1];
     v.rec = g->scene->rec;
+    v.refs = g->refs;
     v.kind = g->kind;
     return v;
 }

@@ -1,14 +1,15 @@
 // sg_trace.hip -- closest-hit 3D-DDA traversal of the uniform and two-level
 // grids and the diffuse bounce generator, for gfx950.
 //
-// One ray per lane.  Rays are SoA (coalesced loads), grid cells / leaves /
-// indices are plain buffers (the reference's texture fetches replaced by
-// explicit clamp-to-edge and zero-out-of-range reads), and each triangle is
-// read from its 64-byte record, whose first 36 bytes are {v0, e1 = v1-v0,
-// e2 = v2-v0}: the edge subtractions are the same binary32 operations the
-// intersector would perform, so results are unchanged and every
-// Moller-Trumbore test needs two 16-byte loads plus one 4-byte load from one
-// aligned record after the index fetch.
+// One ray per lane.  Rays are SoA (coalesced loads), grid cells / leaves
+// are plain buffers (the reference's texture fetches replaced by explicit
+// clamp-to-edge and zero-out-of-range reads), and reference i of a cell is
+// tested from its 48-byte reference record (GridView::refs, written by the
+// build in reference order): {v0, e1 = v1-v0, e2 = v2-v0, triangle id}.  The
+// edge subtractions are the binary32 operations the intersector would
+// perform, so results are unchanged; a test is two 16-byte loads and one
+// 4-byte load at an address known from the cell range alone (no index
+// fetch between the range and the triangle).
 #include <algorithm>
 
 #include "sg_device.hpp"
@@ -74,8 +75,7 @@ constexpr uint32_t kTopWaitMax = 3; // iterations a lane waits for it at most
 // (t, reference), which is what the ordered loop with strict < returns (the
 // smallest distance, the first reference among equal ones).  Returns the
 // key (distance bits << 32 | reference), ~0 when nothing closer than t hit.
-__device__ __attribute__((noinline)) uint64_t coop_tests(const uint32_t* __restrict__ indices,
-                                                         const float4* __restrict__ rec, uint32_t lb, uint32_t le,
+__device__ __attribute__((noinline)) uint64_t coop_tests(const float4* __restrict__ refs, uint32_t lb, uint32_t le,
                                                          float ox, float oy, float oz, float Dx, float Dy, float Dz,
                                                          float t0, uint64_t act) {
     const uint32_t nact = (uint32_t)__popcll(act);
@@ -86,9 +86,8 @@ __device__ __attribute__((noinline)) uint64_t coop_tests(const uint32_t* __restr
     r.t = t0;
     r.best = kMiss;
     for (uint32_t it = lb + rank; it < le; it += nact) {
-        const uint32_t tri = indices[it];
-        mt_test(rec[4 * (size_t)tri], rec[4 * (size_t)tri + 1],
-                reinterpret_cast<const float*>(rec + 4 * (size_t)tri + 2)[0], it, r);
+        const float4* q = refs + 3 * (size_t)it;
+        mt_test(q[0], q[1], q[2].x, it, r);
     }
     // positive distances order as unsigned integers: the smallest key is the
     // smallest t, then the first reference
@@ -224,20 +223,17 @@ SG_D void walk_init_uniform(const GParams& p, const Ray& r, Walk& w) {
 // sequence of operations (nested cell and test loops); for the wave it
 // bounds the iteration count by its busiest lane instead of summing, level
 // by level, the longest loop of any lane, which is what a nested loop costs
-// when the 64 rays of a wave sit in cells of different sizes.  The index of
-// the next reference is loaded one test ahead, so a test waits for one
-// record load.  Lanes whose current cell holds at least kCoopMin references
-// get the whole wave for it (coop_tests).
+// when the 64 rays of a wave sit in cells of different sizes.  Lanes whose
+// current cell holds at least kCoopMin references get the whole wave for it
+// (coop_tests).
 struct Refs {
     uint32_t it, e;    // references [it, e) of the current cell not yet tested
-    uint32_t next;     // indices[it], loaded one test ahead
 };
 
 SG_D void refs_begin(const GridView& g, Refs& q, uint2 rg, Ray& r) {
     q.it = rg.x;
     q.e = rg.y;
     r.tests += rg.y - rg.x;
-    if (q.it < q.e) q.next = g.indices[q.it];
 }
 
 // one test of each lane with references pending, after the wave has taken
@@ -249,7 +245,7 @@ SG_D void refs_step(const GridView& g, Refs& q, Ray& r) {
         while (big) {
             const int L = __ffsll((long long)big) - 1;
             big &= big - 1;
-            const uint64_t m = coop_tests(g.indices, g.rec, (uint32_t)__builtin_amdgcn_readlane((int)q.it, L),
+            const uint64_t m = coop_tests(g.refs, (uint32_t)__builtin_amdgcn_readlane((int)q.it, L),
                                           (uint32_t)__builtin_amdgcn_readlane((int)q.e, L), lane_f(r.o.x, L),
                                           lane_f(r.o.y, L), lane_f(r.o.z, L), lane_f(r.D.x, L), lane_f(r.D.y, L),
                                           lane_f(r.D.z, L), lane_f(r.t, L), act);
@@ -263,10 +259,8 @@ SG_D void refs_step(const GridView& g, Refs& q, Ray& r) {
         }
     }
     if (q.it < q.e) {
-        const uint32_t tri = q.next;
-        if (q.it + 1 < q.e) q.next = g.indices[q.it + 1];
-        mt_test(g.rec[4 * (size_t)tri], g.rec[4 * (size_t)tri + 1],
-                reinterpret_cast<const float*>(g.rec + 4 * (size_t)tri + 2)[0], q.it, r);
+        const float4* rr = g.refs + 3 * (size_t)q.it;
+        mt_test(rr[0], rr[1], rr[2].x, q.it, r);
         ++q.it;
     }
 }
