@@ -204,6 +204,13 @@ static size_t grid_layout(sg_grid* g, char* pool) {
         g->leaves = (uint32_t*)take((size_t)g->cap_C2 * 8 + 16);
     }
     g->refs = (float4*)take((size_t)(two ? g->cap_P2 : g->cap_P1) * 48 + 16);
+    if (two && g->large_path) {
+        g->large.pacc = (uint32_t*)take((size_t)g->cap_P1 * 4 + 16);
+        g->large.prej = (uint32_t*)take((size_t)g->cap_P1 * 4 + 16);
+        g->large.sacc = (uint32_t*)take((size_t)g->cap_P1 * 4 + 16);
+        g->large.srej = (uint32_t*)take((size_t)g->cap_P1 * 4 + 16);
+        g->large.scan_scratch = take(scan_scratch_bytes(g->cap_P1));
+    }
     if (g->p.keep_debug) {
         g->dbg_tri = (uint32_t*)take((size_t)N * 4 + 16);
         g->dbg_leafc = two ? (uint32_t*)take((size_t)g->cap_C1 * 4 + 16) : nullptr;
@@ -309,7 +316,7 @@ static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st, void* cons
     launch_leaf_cells(g->hdr, a, g->ranges1, g->cells1, g->dbg_leafc, g->sums_cells, st);
     mark(5);
     launch_leaf_emit(t, g->hdr, a, g->k1[f1], g->v1[f1], g->ranges1, g->cells1, g->k2[0], g->v2[0], g->refs,
-                     g->dbg_pair, g->sums_leaf, g->leaf_info, st);
+                     g->dbg_pair, g->sums_leaf, g->leaf_info, g->large_path ? &g->large : nullptr, st);
     mark(6);
     launch_leaf_sort_fixup(g->k2[0], g->v2[0], g->k2[1], g->v2[1], g->hdr, g->cap_P2, g->key_bits2, g->rb, g->sort2,
                            st);
@@ -417,6 +424,9 @@ sg_status sg_build(sg_ctx* ctx, sg_scene* scene, const sg_build_params* p, void*
             if ((double)g->cap_P1 < h.need_P1 * (double)slack) { g->cap_P1 = grow(h.need_P1); again = true; }
             if (two && (double)g->cap_C2 < h.need_C2 * (double)slack) { g->cap_C2 = grow(h.need_C2); again = true; }
             if (two && (double)g->cap_P2 < h.need_P2 * (double)slack) { g->cap_P2 = grow(h.need_P2); again = true; }
+            // a tile too large for LDS: lay out the parallel emission and
+            // build again with it (rebuilds then use it too)
+            if (two && h.leaf_sort > 0 && !g->large_path) { g->large_path = true; again = true; }
             if (!again) {
                 *out = g;
                 return SG_OK;

@@ -601,7 +601,8 @@ __global__ __launch_bounds__(kBlock) void k_leaf_count(TriSoA t, GridHeader* h,
                                                        const uint32_t* __restrict__ ranges,
                                                        const uint32_t* __restrict__ cells,
                                                        uint32_t* __restrict__ sums, uint32_t* __restrict__ info,
-                                                       uint32_t T, uint32_t* __restrict__ dbg_counts) {
+                                                       uint32_t T, uint32_t* __restrict__ dbg_counts,
+                                                       uint32_t* __restrict__ pacc, uint32_t* __restrict__ prej) {
     constexpr int kSlots = 4;
     __shared__ uint32_t s_acc[kSlots], s_rej[kSlots];
     const uint32_t P1v = h->P1_valid;
@@ -644,6 +645,10 @@ __global__ __launch_bounds__(kBlock) void k_leaf_count(TriSoA t, GridHeader* h,
         if (dbg_counts) dbg_counts[i] = c;
         uint32_t acc = 0;
         if (c) leaf_visit(L, st, [&](uint32_t, bool ok) { acc += ok ? 1u : 0u; });
+        if (pacc) {   // per-pair totals for the parallel emission of oversize tiles
+            pacc[i] = acc;
+            prej[i] = c - acc;
+        }
         if (tile < T) {
             const uint32_t slot = tile == tile0 ? 0u : tile - tile1 + 1u;
             if (slot < kSlots) {
@@ -948,6 +953,46 @@ __global__ __launch_bounds__(kBlock) void k_leaf_emit_large(TriSoA t, GridHeader
     }
 }
 
+// Oversize tiles with one thread per top pair: pair i of tile t writes its
+// accepted pairs from pre_acc(t) + (sacc[i] - sacc[first pair of t]) and its
+// rejected ones likewise (sacc / srej: exclusive scans of the per-pair totals
+// of k_leaf_count), so the output is k_leaf_emit_large's, in parallel -- a
+// top cell of 500k pairs (a dense cluster) no longer runs on one block.
+__global__ __launch_bounds__(kBlock) void k_leaf_emit_par(TriSoA t, GridHeader* h, BuildArgs a,
+                                                          const uint32_t* __restrict__ top_keys,
+                                                          const uint32_t* __restrict__ top_vals,
+                                                          const uint32_t* __restrict__ ranges,
+                                                          const uint32_t* __restrict__ cells,
+                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                          const uint32_t* __restrict__ sums,
+                                                          const uint32_t* __restrict__ info, uint32_t T,
+                                                          const uint32_t* __restrict__ sacc,
+                                                          const uint32_t* __restrict__ srej) {
+    const LeafParams L = read_leaf_params(h);
+    const uint32_t P1v = min(h->P1_valid, a.cap_P1);
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < P1v; i += gridDim.x * kBlock) {
+        const uint32_t key = top_keys[i];
+        if (key >= L.C1) continue;
+        const uint32_t tile = reinterpret_cast<const uint2*>(ranges)[key].x / kLeafTile;
+        LeafTilePlan q;
+        if (tile >= T || !leaf_tile_plan(a, sums, info, T, tile, q) || q.staged) continue;
+        if (i == q.a0) h->leaf_sort = q.tot_acc;
+        uint32_t pa = q.pre_acc + (sacc[i] - sacc[q.a0]);
+        uint32_t pr = q.tot_acc + q.pre_rej + (srej[i] - srej[q.a0]);
+        const uint32_t tri = top_vals[i];
+        LeafSetup s;
+        const uint32_t c = leaf_pair(t, L, h, key, tri, cells, s);
+        if (c)
+            leaf_visit(L, s, [&](uint32_t k2, bool ok) {
+                const uint32_t idx = ok ? pa++ : pr++;
+                if (idx < a.cap_P2) {
+                    keys[idx] = ok ? k2 : L.C2;
+                    vals[idx] = tri;
+                }
+            });
+    }
+}
+
 // The global stable sort of the accepted leaf pairs works in two buffers;
 // when its pass count is odd this copies its result back to the first one.
 // n = GridHeader::leaf_sort, zero when every tile was sorted in LDS.
@@ -1010,17 +1055,27 @@ void launch_leaf_cells(GridHeader* h, const BuildArgs& a, const uint32_t* ranges
 
 void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const uint32_t* top_keys,
                       const uint32_t* top_vals, const uint32_t* ranges, const uint32_t* cells, uint32_t* keys,
-                      uint32_t* vals, float4* refs, uint32_t* dbg, uint32_t* sums, uint32_t* info, hipStream_t st) {
+                      uint32_t* vals, float4* refs, uint32_t* dbg, uint32_t* sums, uint32_t* info,
+                      const LeafLarge* large, hipStream_t st) {
     const uint32_t T = (uint32_t)leaf_tiles(a.cap_P1);
     const uint32_t blocks = (a.cap_P1 + kTile - 1) / kTile;
     if (blocks)
         hipLaunchKernelGGL(k_leaf_count, dim3(blocks), dim3(kBlock), 0, st, t, h, top_keys, top_vals, ranges, cells,
-                           sums, info, T, dbg);
+                           sums, info, T, dbg, large ? large->pacc : nullptr, large ? large->prej : nullptr);
     hipLaunchKernelGGL(k_leaf_scan, dim3(1), dim3(1024), 0, st, h, a, sums, T);
     hipLaunchKernelGGL(k_leaf_emit, dim3(T), dim3(kLeafBlock), 0, st, t, h, a, top_keys, top_vals, cells, keys, vals,
                        refs, sums, info, T);
-    hipLaunchKernelGGL(k_leaf_emit_large, dim3(T), dim3(kBlock), 0, st, t, h, a, top_keys, top_vals, cells, keys,
-                       vals, sums, info, T);
+    if (large) {
+        launch_scan(large->pacc, large->sacc, a.cap_P1, false, large->scan_scratch, st);
+        launch_scan(large->prej, large->srej, a.cap_P1, false, large->scan_scratch, st);
+        uint32_t pb = (a.cap_P1 + kBlock - 1) / kBlock;
+        pb = pb < 1 ? 1 : (pb > 4096 ? 4096 : pb);
+        hipLaunchKernelGGL(k_leaf_emit_par, dim3(pb), dim3(kBlock), 0, st, t, h, a, top_keys, top_vals, ranges, cells,
+                           keys, vals, sums, info, T, large->sacc, large->srej);
+    } else {
+        hipLaunchKernelGGL(k_leaf_emit_large, dim3(T), dim3(kBlock), 0, st, t, h, a, top_keys, top_vals, cells, keys,
+                           vals, sums, info, T);
+    }
 }
 
 void launch_leaf_sort_fixup(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, GridHeader* h, uint32_t cap,

@@ -44,6 +44,10 @@ struct sg_grid {
     uint32_t *k2[2] = {nullptr, nullptr}, *v2[2] = {nullptr, nullptr};
     uint32_t* leaves = nullptr;
     float4* refs = nullptr;   // per-reference records (launch_ref_records)
+    // parallel emission of oversize leaf tiles, laid out once a build of this
+    // grid has met one (GridHeader::leaf_sort > 0)
+    bool large_path = false;
+    LeafLarge large{};
     uint32_t *dbg_tri = nullptr, *dbg_leafc = nullptr, *dbg_pair = nullptr;
     uint32_t *sums_top = nullptr, *sums_cells = nullptr, *sums_leaf = nullptr;   // per-tile sums
     uint32_t* leaf_info = nullptr;                                               // per leaf tile bounds

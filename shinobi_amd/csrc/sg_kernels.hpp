@@ -102,10 +102,18 @@ void launch_leaf_cells(GridHeader* h, const BuildArgs& a, const uint32_t* ranges
 size_t leaf_tiles(uint32_t n);
 size_t leaf_sums_words(uint32_t n);
 size_t leaf_info_words(uint32_t n);
+// Buffers of the parallel emission of tiles too large for LDS (per top pair:
+// accepted / rejected leaf-pair totals and their exclusive scans).  Without
+// them such a tile is emitted by one block.
+struct LeafLarge {
+    uint32_t *pacc, *prej, *sacc, *srej;
+    void* scan_scratch;   // scan_scratch_bytes(cap_P1)
+};
 // refs: reference records of the accepted pairs of LDS-sorted tiles
 void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const uint32_t* top_keys,
                       const uint32_t* top_vals, const uint32_t* ranges, const uint32_t* cells, uint32_t* keys,
-                      uint32_t* vals, float4* refs, uint32_t* dbg, uint32_t* sums, uint32_t* info, hipStream_t st);
+                      uint32_t* vals, float4* refs, uint32_t* dbg, uint32_t* sums, uint32_t* info,
+                      const LeafLarge* large, hipStream_t st);
 // global stable sort of the leaf pairs of tiles too large for LDS (no work
 // when there are none); the result stays in (k0, v0)
 void launch_leaf_sort_fixup(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, GridHeader* h, uint32_t cap,

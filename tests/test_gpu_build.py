@@ -84,11 +84,39 @@ def test_twolevel_heightfield2m_bit_exact(ctx, oracle):
 
 
 def test_twolevel_cluster_bit_exact(ctx, oracle):
+    """the cluster's dense top cell is a leaf tile too large for LDS: its
+    pairs take the parallel emission and the global leaf sort, in the build
+    and in every rebuild"""
     w = W.cluster(n_small=60_000, n_large=400, res_x=32, res_y=32)
     og = oracle.build_twolevel(w.scene, **w.oracle)
     _, g = _gpu_build(ctx, w.scene, 1, **w.build)
     info = g.info()
     assert info.status == og.status
+    _assert_top_level(g, og)
+    _assert_leaf_level(g, og)
+    g.rebuild()
+    _assert_top_level(g, og)
+    _assert_leaf_level(g, og)
+
+
+def test_rebuild_into_an_oversize_tile(ctx, oracle):
+    """a grid built without oversize leaf tiles and rebuilt after 5,000 of its
+    triangles moved into one top cell: the one-block emission of such tiles
+    (used until a build of the grid has met one) gives the oracle's grid"""
+    a = sa.gen_soup(71, 20_000, 0.01)
+    b = a.copy()
+    k = 5_000
+    c = np.float32(0.301)
+    for j in range(3):   # shrink triangles 0..k-1 around (0.301, 0.301, 0.301)
+        b[3 * j:3 * j + 3, :k] = c + (a[3 * j:3 * j + 3, :k] - np.float32(0.5)) * np.float32(0.004)
+    scene, g = _gpu_build(ctx, a, 1, lam=5.0, lambda_leaf=2.0)
+    res = tuple(g.info().res)
+    scene.upload_vertices(b)
+    g.rebuild()
+    og = oracle.build_twolevel(b, lam=5.0, top_divisor=6, lambda_leaf=2.0, res_override=res)
+    info = g.info()
+    assert info.status == og.status == 0
+    assert tuple(info.res) == tuple(og.res)
     _assert_top_level(g, og)
     _assert_leaf_level(g, og)
 
