@@ -518,6 +518,7 @@ def main() -> int:
                    "primary_order": f"{PRIMARY_TILE}x{PRIMARY_TILE} pixel tiles",
                    "n_tris": n_tris, "primary_rays": n_rays, "bounce_rays": m, "res_top": list(info.res),
                    "C1": info.C1, "P1": info.P1, "C2": info.C2, "P2": info.P2},
+        "grid_bytes": info.device_bytes,
         "build_ms": round(mean(build_ms), 4),
         "build_ms_median": round(float(np.median(build_ms)), 4),
         "trace_primary_ms": round(mean(tp_ms), 4),
