@@ -147,11 +147,20 @@ struct TraceItem {
     uint32_t n;
     uint32_t chunk0;
 };
-// persistent traversal of n_items items (device table) in one launch; the
-// counter (one device word) is reset by a kernel first, so the pair of
-// launches can be captured in a graph and replayed
-void launch_trace_items(const TraceItem* items, uint32_t n_items, uint32_t total_chunks, uint32_t* counter,
-                        bool shadow, hipStream_t st);
+// A table of items split by grid kind: uniform items first, then two-level
+// ones, each part with its own chunk numbering from 0.
+struct ItemSplit {
+    uint32_t n[2] = {0, 0};        // items of each kind
+    uint32_t chunks[2] = {0, 0};   // 64-ray chunks of each kind
+};
+// lay out n host items in `out` as described by ItemSplit (chunk0 renumbered)
+ItemSplit split_items(const TraceItem* in, uint32_t n, TraceItem* out);
+// persistent traversal of the items (device table laid out by split_items):
+// one launch per kind present, each kernel holding only its kind's walk (and
+// so its registers).  counter: two device words, reset by a kernel first, so
+// the launches can be captured in a graph and replayed.
+void launch_trace_items(const TraceItem* items, const ItemSplit& split, uint32_t* counter, bool shadow,
+                        hipStream_t st);
 void init_trace_items();
 // closest hit, or with shadow the any-hit traversal that stops at t < 0.9999
 // (block: threads per block, 64, 128 or 256)

@@ -41,7 +41,7 @@ struct sg_batch {
     TraceItem* dev = nullptr;      // device table
     uint32_t* counter = nullptr;   // device chunk counter
     hipEvent_t uploaded = nullptr; // the last upload out of `pinned`
-    uint32_t dev_items = 0, dev_chunks = 0;
+    ItemSplit dev_split;           // layout of the device table
 };
 
 extern "C" {
@@ -109,13 +109,9 @@ sg_status sg_batch_commit(sg_batch* b, void* stream) {
     HIPCHK(hipSetDevice(b->ctx->device));
     HIPCHK(hipEventSynchronize(b->uploaded));   // the staging buffer is free again
     const size_t bytes = sizeof(TraceItem) * b->items.size();
-    if (bytes) {
-        memcpy(b->pinned, b->items.data(), bytes);
-        HIPCHK(hipMemcpyAsync(b->dev, b->pinned, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
-    }
+    b->dev_split = split_items(b->items.data(), (uint32_t)b->items.size(), b->pinned);
+    if (bytes) HIPCHK(hipMemcpyAsync(b->dev, b->pinned, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
     HIPCHK(hipEventRecord(b->uploaded, (hipStream_t)stream));
-    b->dev_items = (uint32_t)b->items.size();
-    b->dev_chunks = b->next_chunk;
     return SG_OK;
 }
 
@@ -123,7 +119,7 @@ sg_status sg_batch_launch(sg_batch* b, int shadow, void* stream) {
     if (!b) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(b->ctx->device));
     clear_last_error();
-    launch_trace_items(b->dev, b->dev_items, b->dev_chunks, b->counter, shadow != 0, (hipStream_t)stream);
+    launch_trace_items(b->dev, b->dev_split, b->counter, shadow != 0, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return SG_OK;
 }
@@ -415,6 +411,7 @@ sg_status launch_wave(sg_server* s, Slot& sl) {
         oo += align_up((size_t)sl.reqs[k]->n * 4, kAlign) * 2;
     }
     uint32_t chunk = 0;
+    std::vector<TraceItem> wave_items(plain ? 0 : sl.reqs.size());
     for (size_t k = 0; k < sl.reqs.size(); ++k) {
         const Request* r = sl.reqs[k];
         const size_t nb = (size_t)r->n * 4, a = align_up(nb, kAlign);
@@ -440,13 +437,14 @@ sg_status launch_wave(sg_server* s, Slot& sl) {
             launch_trace(it.g, it.rays, it.n, nullptr, it.prim, it.t, nullptr, false, se.grid[gi]->p.trace_block,
                          sl.st);
         } else {
-            sl.pinned_items[k] = it;
+            wave_items[k] = it;
         }
     }
     if (!plain) {
+        const ItemSplit split = split_items(wave_items.data(), (uint32_t)sl.reqs.size(), sl.pinned_items);
         HIPCHK(hipMemcpyAsync(sl.dev_items, sl.pinned_items, sizeof(TraceItem) * sl.reqs.size(),
                               hipMemcpyHostToDevice, sl.st));
-        launch_trace_items(sl.dev_items, (uint32_t)sl.reqs.size(), chunk, sl.counter, false, sl.st);
+        launch_trace_items(sl.dev_items, split, sl.counter, false, sl.st);
     }
     HIPCHK(hipGetLastError());
     for (size_t k = 0; k < sl.reqs.size(); ++k) {

@@ -468,10 +468,6 @@ __global__ __launch_bounds__(kBlock) void k_trace(GridView g, RaySoA rays, uint3
     trace_one<KIND, STATS, SHADOW>(g, rays, i, prim, tout, stats);
 }
 
-__global__ void k_set_counter(uint32_t* p) {
-    if (threadIdx.x == 0) *p = 0u;
-}
-
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim, float* t,
                   uint32_t* stats, bool shadow, int block, hipStream_t st) {
     if (n == 0) return;
@@ -494,11 +490,13 @@ void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t
 // Persistent traversal of a table of (grid, ray batch) items: every wave
 // takes the next 64-ray chunk of the concatenated batches from one counter
 // (one device atomic per 64 rays, far below the atomic unit's rate) and
-// traces it against that chunk's grid, so batches of any size and grids of
-// either kind share one launch and the load balances across the chip.
+// traces it against that chunk's grid, so batches of any size share one
+// launch and the load balances across the chip.  One kernel per grid kind:
+// a kernel that could run either walk would hold the registers of the
+// larger one (104 VGPRs against 67 for the uniform walk alone).
+template <int KIND, bool SHADOW>
 __global__ __launch_bounds__(kBlock) void k_trace_items(const TraceItem* __restrict__ items, uint32_t n_items,
-                                                        uint32_t total_chunks, uint32_t* __restrict__ counter,
-                                                        int shadow) {
+                                                        uint32_t total_chunks, uint32_t* __restrict__ counter) {
     const int l = lane_id();
     for (;;) {
         uint32_t c = 0;
@@ -514,43 +512,68 @@ __global__ __launch_bounds__(kBlock) void k_trace_items(const TraceItem* __restr
         }
         const TraceItem& it = items[lo];
         const uint32_t i = (c - it.chunk0) * 64u + (uint32_t)l;
-        if (i < it.n) {
-            if (it.g.kind == 0) {
-                if (shadow) trace_one<0, false, true>(it.g, it.rays, i, it.prim, it.t, nullptr);
-                else trace_one<0, false, false>(it.g, it.rays, i, it.prim, it.t, nullptr);
-            } else {
-                if (shadow) trace_one<1, false, true>(it.g, it.rays, i, it.prim, it.t, nullptr);
-                else trace_one<1, false, false>(it.g, it.rays, i, it.prim, it.t, nullptr);
-            }
-        }
+        if (i < it.n) trace_one<KIND, false, SHADOW>(it.g, it.rays, i, it.prim, it.t, nullptr);
     }
 }
 
-static int g_item_blocks = 0;
-
-// resident blocks of k_trace_items on the current device (queried once,
-// outside any stream capture)
-void init_trace_items() {
-    if (g_item_blocks != 0) return;
-    int dev = 0, cus = 0, per = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_trace_items), kBlock,
-                                                        0);
-    g_item_blocks = std::max(1, cus) * std::max(1, per);
+__global__ void k_set_counter(uint32_t* p) {
+    if (threadIdx.x < 2) p[threadIdx.x] = 0u;
 }
 
-void launch_trace_items(const TraceItem* items, uint32_t n_items, uint32_t total_chunks, uint32_t* counter,
-                        bool shadow, hipStream_t st) {
+static int g_item_blocks[2][2] = {{0, 0}, {0, 0}};
+
+template <int KIND, bool SHADOW>
+static int item_blocks(int cus) {
+    int per = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per, reinterpret_cast<const void*>(k_trace_items<KIND, SHADOW>), kBlock, 0);
+    return std::max(1, cus) * std::max(1, per);
+}
+
+// resident blocks of each k_trace_items on the current device (queried once,
+// outside any stream capture)
+void init_trace_items() {
+    if (g_item_blocks[0][0] != 0) return;
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    g_item_blocks[0][0] = item_blocks<0, false>(cus);
+    g_item_blocks[0][1] = item_blocks<0, true>(cus);
+    g_item_blocks[1][0] = item_blocks<1, false>(cus);
+    g_item_blocks[1][1] = item_blocks<1, true>(cus);
+}
+
+ItemSplit split_items(const TraceItem* in, uint32_t n, TraceItem* out) {
+    ItemSplit s;
+    for (uint32_t i = 0; i < n; ++i) s.n[in[i].g.kind == 0 ? 0 : 1]++;
+    uint32_t at[2] = {0, s.n[0]};
+    for (uint32_t i = 0; i < n; ++i) {
+        const int k = in[i].g.kind == 0 ? 0 : 1;
+        TraceItem it = in[i];
+        it.chunk0 = s.chunks[k];
+        s.chunks[k] += (it.n + 63u) / 64u;
+        out[at[k]++] = it;
+    }
+    return s;
+}
+
+void launch_trace_items(const TraceItem* items, const ItemSplit& split, uint32_t* counter, bool shadow,
+                        hipStream_t st) {
     hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, st, counter);
-    if (n_items == 0 || total_chunks == 0) return;
     init_trace_items();
-    const uint32_t waves_needed = total_chunks;
-    uint32_t blocks = (uint32_t)g_item_blocks;
-    const uint32_t max_blocks = (waves_needed + kBlock / 64 - 1) / (kBlock / 64);
-    if (blocks > max_blocks) blocks = max_blocks;
-    hipLaunchKernelGGL(k_trace_items, dim3(blocks), dim3(kBlock), 0, st, items, n_items, total_chunks, counter,
-                       shadow ? 1 : 0);
+    for (int k = 0; k < 2; ++k) {
+        if (split.n[k] == 0 || split.chunks[k] == 0) continue;
+        const TraceItem* part = items + (k ? split.n[0] : 0);
+        uint32_t blocks = (uint32_t)g_item_blocks[k][shadow ? 1 : 0];
+        const uint32_t max_blocks = (split.chunks[k] + kBlock / 64 - 1) / (kBlock / 64);
+        if (blocks > max_blocks) blocks = max_blocks;
+#define SG_ITEMS(K, H) \
+    hipLaunchKernelGGL((k_trace_items<K, H>), dim3(blocks), dim3(kBlock), 0, st, part, split.n[k], split.chunks[k], \
+                       counter + k)
+        if (k == 0) { if (shadow) SG_ITEMS(0, true); else SG_ITEMS(0, false); }
+        else { if (shadow) SG_ITEMS(1, true); else SG_ITEMS(1, false); }
+#undef SG_ITEMS
+    }
 }
 
 // ------------------------------------------------------------------ bounce
