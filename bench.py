@@ -271,7 +271,7 @@ def batched_bench(sa, ctx, n_scenes: int = 64, reps: int = 5) -> dict:
     [1e3, 5e5] with edge scale 0.01*N^(-1/3), each with log-uniform
     [1e4, 2e6] random rays; per-scene density, radix width and block size
     from the autotuner.  One captured graph rebuilds every grid and traces
-    every batch in one persistent launch; the plain path runs the same
+    every batch in one multi-batch launch; the plain path runs the same
     rebuilds and one trace launch per scene."""
     rng = np.random.default_rng(4)
     ns = _log_uniform(rng, 1_000, 500_000, n_scenes)

@@ -219,12 +219,12 @@ sg_status sg_event_record(sg_ctx *ctx, void *ev, void *stream);
 sg_status sg_event_elapsed_ms(sg_ctx *ctx, void *ev_start, void *ev_end, float *ms);
 
 /* ------------------------------------------------------------ batching --- */
-/* Many (grid, ray batch) items traced by ONE persistent launch: the
- * reference's persistent trace kernel (src/Engine/RayTracingKernels.h:37-136,
- * per-warp pool of ray ids from a global counter, :76) generalised from one
- * ray buffer to a table of batches against different grids.  Every wave
- * takes the next 64 rays of the concatenated batches; grids of both kinds mix
- * in one launch.  Results equal sg_trace_closest / sg_trace_shadow per item. */
+/* Many (grid, ray batch) items traced by one launch per grid kind: the
+ * reference's trace kernel (src/Engine/RayTracingKernels.h:37-136, a
+ * persistent pool of ray ids from a global counter, :76) generalised from one
+ * ray buffer to a table of batches against different grids.  Wave w traces
+ * the w-th 64 rays of the concatenated batches; grids of both kinds mix in
+ * one table.  Results equal sg_trace_closest / sg_trace_shadow per item. */
 typedef struct sg_batch sg_batch;
 sg_status sg_batch_create(sg_ctx *ctx, uint32_t max_items, sg_batch **out);
 sg_status sg_batch_destroy(sg_batch *batch);
@@ -270,8 +270,8 @@ sg_status sg_autotune(sg_ctx *ctx, sg_scene *scene, const sg_rays *sample, uint3
  * Requests (host SoA rays in, host prim/t out) are admitted while free slots
  * and HBM budget (device bytes for request rays and results) allow.
  * SG_SERVE_BATCHED: continuous batching -- every request that is admitted
- * while earlier waves run joins the next wave; a wave is one persistent
- * multi-batch launch (sg_batch_*) with its copies, up to max_waves waves are
+ * while earlier waves run joins the next wave; a wave is one multi-batch
+ * launch (as sg_batch_*) with its copies, up to max_waves waves are
  * in flight on separate streams, so copies and traversal of different waves
  * overlap.  SG_SERVE_PLAIN: the per-request path of the reference -- one
  * request at a time: copy in, one trace launch, copy out, synchronise.

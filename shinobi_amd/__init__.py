@@ -499,7 +499,7 @@ def sort_pairs(ctx: Context, keys: np.ndarray, vals: np.ndarray, key_bits: int =
 
 # ------------------------------------------------------------------ batching
 class Batch:
-    """Many (grid, ray batch) items traced by one persistent launch
+    """Many (grid, ray batch) items traced by one launch per grid kind
     (sg_batch_*)."""
 
     def __init__(self, ctx: Context, max_items: int):

@@ -155,13 +155,11 @@ struct ItemSplit {
 };
 // lay out n host items in `out` as described by ItemSplit (chunk0 renumbered)
 ItemSplit split_items(const TraceItem* in, uint32_t n, TraceItem* out);
-// persistent traversal of the items (device table laid out by split_items):
-// one launch per kind present, each kernel holding only its kind's walk (and
-// so its registers).  counter: two device words, reset by a kernel first, so
-// the launches can be captured in a graph and replayed.
-void launch_trace_items(const TraceItem* items, const ItemSplit& split, uint32_t* counter, bool shadow,
-                        hipStream_t st);
-void init_trace_items();
+// traversal of the items (device table laid out by split_items): one launch
+// per kind present, one wave per 64-ray chunk, each kernel holding only its
+// kind's walk (and so its registers); enqueues kernels only, so the
+// launches can be captured in a graph and replayed
+void launch_trace_items(const TraceItem* items, const ItemSplit& split, bool shadow, hipStream_t st);
 // closest hit, or with shadow the any-hit traversal that stops at t < 0.9999
 // (block: threads per block, 64, 128 or 256)
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim,

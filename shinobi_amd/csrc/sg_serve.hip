@@ -3,9 +3,9 @@
 //
 // The reference traces one ray buffer per persistent launch
 // (src/Engine/RayTracingKernels.h:37-136) and its engines call it once per
-// frame.  Here a table of (grid, ray batch) items is traced by one persistent
-// launch (k_trace_items), which is what both the batched multi-scene runs and
-// the serving loop build on.
+// frame.  Here a table of (grid, ray batch) items is traced by one launch per
+// grid kind (k_trace_items), which is what both the batched multi-scene runs
+// and the serving loop build on.
 #include <string.h>
 
 #include <algorithm>
@@ -39,7 +39,6 @@ struct sg_batch {
     uint32_t next_chunk = 0;
     TraceItem* pinned = nullptr;   // staging of the table for the upload
     TraceItem* dev = nullptr;      // device table
-    uint32_t* counter = nullptr;   // device chunk counter
     hipEvent_t uploaded = nullptr; // the last upload out of `pinned`
     ItemSplit dev_split;           // layout of the device table
 };
@@ -53,13 +52,11 @@ sg_status sg_batch_create(sg_ctx* ctx, uint32_t max_items, sg_batch** out) {
     b->ctx = ctx;
     b->cap = max_items;
     b->items.reserve(max_items);
-    init_trace_items();
     if (hipHostMalloc((void**)&b->pinned, sizeof(TraceItem) * max_items, hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void**)&b->dev, sizeof(TraceItem) * max_items) != hipSuccess ||
-        hipMalloc((void**)&b->counter, 256) != hipSuccess || hipEventCreate(&b->uploaded) != hipSuccess) {
+        hipEventCreate(&b->uploaded) != hipSuccess) {
         if (b->pinned) hipHostFree(b->pinned);
         if (b->dev) hipFree(b->dev);
-        if (b->counter) hipFree(b->counter);
         delete b;
         return SG_ERR_OUT_OF_MEMORY;
     }
@@ -74,7 +71,6 @@ sg_status sg_batch_destroy(sg_batch* b) {
     hipEventDestroy(b->uploaded);
     hipHostFree(b->pinned);
     hipFree(b->dev);
-    hipFree(b->counter);
     delete b;
     return SG_OK;
 }
@@ -119,7 +115,7 @@ sg_status sg_batch_launch(sg_batch* b, int shadow, void* stream) {
     if (!b) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(b->ctx->device));
     clear_last_error();
-    launch_trace_items(b->dev, b->dev_split, b->counter, shadow != 0, (hipStream_t)stream);
+    launch_trace_items(b->dev, b->dev_split, shadow != 0, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return SG_OK;
 }
@@ -307,7 +303,6 @@ struct Slot {
     hipEvent_t ev = nullptr;
     TraceItem* pinned_items = nullptr;
     TraceItem* dev_items = nullptr;
-    uint32_t* counter = nullptr;
     bool busy = false;
     std::vector<Request*> reqs;
     size_t off = 0, bytes = 0, rays_bytes = 0;   // arena region of the wave
@@ -444,7 +439,7 @@ sg_status launch_wave(sg_server* s, Slot& sl) {
         const ItemSplit split = split_items(wave_items.data(), (uint32_t)sl.reqs.size(), sl.pinned_items);
         HIPCHK(hipMemcpyAsync(sl.dev_items, sl.pinned_items, sizeof(TraceItem) * sl.reqs.size(),
                               hipMemcpyHostToDevice, sl.st));
-        launch_trace_items(sl.dev_items, split, sl.counter, false, sl.st);
+        launch_trace_items(sl.dev_items, split, false, sl.st);
     }
     HIPCHK(hipGetLastError());
     for (size_t k = 0; k < sl.reqs.size(); ++k) {
@@ -577,15 +572,13 @@ sg_status sg_server_create(sg_ctx* ctx, const sg_server_config* cfg, sg_server**
              hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) == hipSuccess &&
              hipHostMalloc((void**)&sl.pinned_items, sizeof(TraceItem) * s->cfg.max_requests,
                            hipHostMallocDefault) == hipSuccess &&
-             hipMalloc((void**)&sl.dev_items, sizeof(TraceItem) * s->cfg.max_requests) == hipSuccess &&
-             hipMalloc((void**)&sl.counter, 256) == hipSuccess;
+             hipMalloc((void**)&sl.dev_items, sizeof(TraceItem) * s->cfg.max_requests) == hipSuccess;
     }
     if (!ok) {
         s->stop = true;
         sg_server_destroy(s);
         return SG_ERR_OUT_OF_MEMORY;
     }
-    init_trace_items();
     s->worker = std::thread(serve_loop, s);
     *out = s;
     return SG_OK;
@@ -607,7 +600,6 @@ sg_status sg_server_destroy(sg_server* s) {
         if (sl.ev) hipEventDestroy(sl.ev);
         if (sl.pinned_items) hipHostFree(sl.pinned_items);
         if (sl.dev_items) hipFree(sl.dev_items);
-        if (sl.counter) hipFree(sl.counter);
     }
     for (SceneEntry& se : s->scenes) {
         if (!se.dynamic) continue;
