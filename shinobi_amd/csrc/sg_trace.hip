@@ -113,7 +113,6 @@ SG_D void mt_test(const float4 q0, const float4 q1, const float q2, uint32_t it,
     }
 }
 
-constexpr uint32_t kCoopMin = 64;   // references from which the wave shares a cell's tests
 constexpr int kTopBatch = 16;       // lanes waiting for a top step that make the wave take it
 constexpr uint32_t kTopWaitMax = 3; // iterations a lane waits for it at most
 
@@ -140,7 +139,7 @@ __device__ __attribute__((noinline)) uint64_t coop_tests(gptr<const float4> refs
     // smallest t, then the first reference
     const uint64_t key = r.best == kMiss ? ~0ull : ((uint64_t)__float_as_uint(r.t) << 32) | r.best;
     uint64_t m = ~0ull;
-    for (uint64_t a = act; a; a &= a - 1) {
+    for (uint64_t a = act & __ballot(key != ~0ull); a; a &= a - 1) {   // lanes that found a closer hit
         const int j = __ffsll((long long)a) - 1;
         const uint64_t kj = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(key >> 32), j) << 32) |
                             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, j);
@@ -283,8 +282,8 @@ SG_D void walk_init_uniform(const GParams& p, const Ray& r, Walk& w) {
 // bounds the iteration count by its busiest lane instead of summing, level
 // by level, the longest loop of any lane, which is what a nested loop costs
 // when the 64 rays of a wave sit in cells of different sizes.  Lanes whose
-// current cell holds at least kCoopMin references get the whole wave for it
-// (coop_tests).
+// current cell holds at least coop_threshold() references get the whole
+// wave for it (coop_tests).
 struct Refs {
     uint32_t it, e;    // references [it, e) of the current cell not yet tested
 };
@@ -295,10 +294,19 @@ SG_D void refs_begin(const GView& g, Refs& q, uint2 rg, Ray& r) {
     r.tests += rg.y - rg.x;
 }
 
-// the wave takes over the dense cells: every lane with at least kCoopMin
-// references pending has them all tested by the wave (coop_tests)
-SG_D void refs_coop(const GView& g, Refs& q, Ray& r) {
-    uint64_t big = __ballot(q.e - q.it >= kCoopMin && q.it < q.e);
+// How many references a cell needs before the wave tests it together.  A
+// wave lives as long as its longest ray, whose tests run one per iteration,
+// each a memory latency; once most lanes are done, the idle lanes can test
+// a remaining ray's cell in one round instead.  Sharing costs the other
+// active lanes one round, so the threshold falls with the number of active
+// lanes n: n/2 + 2, so 34 for a full wave and 2 for a lone ray (measured
+// on the height field against n + 1, n/3 + 2, n/4 + 2 and a fixed 64).
+SG_D uint32_t coop_threshold(bool active) { return (uint32_t)__popcll(__ballot(active)) / 2u + 2u; }
+
+// The wave takes over the dense cells: every lane with at least `thr`
+// references pending has them all tested by the wave (coop_tests).
+SG_D void refs_coop(const GView& g, Refs& q, Ray& r, uint32_t thr) {
+    uint64_t big = __ballot(q.e - q.it >= thr && q.it < q.e);
     if (big) {
         const uint64_t act = __ballot(true);
         while (big) {
@@ -351,7 +359,7 @@ SG_D void walk_uniform(const GView& g, const GParams& p, Ray& r, Walk& w, uint32
     if (w.active && !(w.guard++ < cap)) w.active = false;
     if (w.active) refs_begin(g, q, load_cell(g, p, w.cell), r);
     while (__ballot(w.active)) {
-        refs_coop(g, q, r);
+        refs_coop(g, q, r, coop_threshold(w.active));
         const bool test = q.it < q.e;
         RefRec x;
         if (test) ref_load(g, q, x);
@@ -467,7 +475,7 @@ SG_D void walk_twolevel(const GView& g, const GParams& p, Ray& r, Walk& w, uint3
     bool top_wait = false;
     uint32_t waited = 0;
     while (__ballot(w.active)) {
-        refs_coop(g, q, r);
+        refs_coop(g, q, r, coop_threshold(w.active));
         const bool test = q.it < q.e;
         RefRec x;
         if (test) ref_load(g, q, x);
