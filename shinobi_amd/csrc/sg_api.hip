@@ -203,7 +203,7 @@ static size_t grid_layout(sg_grid* g, char* pool) {
         }
         g->leaves = (uint32_t*)take((size_t)g->cap_C2 * 8 + 16);
     }
-    g->refs = (float4*)take((size_t)(two ? g->cap_P2 : g->cap_P1) * 48 + 16);
+    g->refs = (float*)take((size_t)(two ? g->cap_P2 : g->cap_P1) * 36 + 16);
     if (two && g->large_path) {
         g->large.pacc = (uint32_t*)take((size_t)g->cap_P1 * 4 + 16);
         g->large.prej = (uint32_t*)take((size_t)g->cap_P1 * 4 + 16);

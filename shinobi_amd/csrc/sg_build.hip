@@ -842,7 +842,7 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* 
                                                          const uint32_t* __restrict__ top_vals,
                                                          const uint32_t* __restrict__ cells,
                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                         float4* __restrict__ refs,
+                                                         float* __restrict__ refs,
                                                          const uint32_t* __restrict__ sums,
                                                          const uint32_t* __restrict__ info, uint32_t T) {
     constexpr int NW = kLeafBlock / 64;
@@ -1055,7 +1055,7 @@ void launch_leaf_cells(GridHeader* h, const BuildArgs& a, const uint32_t* ranges
 
 void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const uint32_t* top_keys,
                       const uint32_t* top_vals, const uint32_t* ranges, const uint32_t* cells, uint32_t* keys,
-                      uint32_t* vals, float4* refs, uint32_t* dbg, uint32_t* sums, uint32_t* info,
+                      uint32_t* vals, float* refs, uint32_t* dbg, uint32_t* sums, uint32_t* info,
                       const LeafLarge* large, hipStream_t st) {
     const uint32_t T = (uint32_t)leaf_tiles(a.cap_P1);
     const uint32_t blocks = (a.cap_P1 + kTile - 1) / kTile;
@@ -1094,7 +1094,7 @@ void launch_leaf_sort_fixup(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* 
 __global__ __launch_bounds__(kBlock) void k_ref_records(const uint32_t* __restrict__ indices, const uint32_t* d_valid,
                                                         const uint32_t* d_gate, uint32_t cap,
                                                         const float4* __restrict__ rec, uint32_t n_tris,
-                                                        float4* __restrict__ refs) {
+                                                        float* __restrict__ refs) {
     if (d_gate && *d_gate == 0) return;
     const uint32_t n = min(*d_valid, cap);
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
@@ -1102,7 +1102,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_records(const uint32_t* __restri
 }
 
 void launch_ref_records(const uint32_t* indices, const uint32_t* d_valid, const uint32_t* d_gate, uint32_t cap,
-                        const float4* rec, uint32_t n_tris, float4* refs, hipStream_t st) {
+                        const float4* rec, uint32_t n_tris, float* refs, hipStream_t st) {
     uint32_t blocks = (cap + kBlock * 4 - 1) / (kBlock * 4);
     blocks = blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks);
     hipLaunchKernelGGL(k_ref_records, dim3(blocks), dim3(kBlock), 0, st, indices, d_valid, d_gate, cap, rec, n_tris,

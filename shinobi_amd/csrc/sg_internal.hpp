@@ -43,7 +43,7 @@ struct sg_grid {
     uint32_t *ranges1 = nullptr, *cells1 = nullptr;
     uint32_t *k2[2] = {nullptr, nullptr}, *v2[2] = {nullptr, nullptr};
     uint32_t* leaves = nullptr;
-    float4* refs = nullptr;   // per-reference records (launch_ref_records)
+    float* refs = nullptr;    // per-reference records (write_ref_record)
     // parallel emission of oversize leaf tiles, laid out once a build of this
     // grid has met one (GridHeader::leaf_sort > 0)
     bool large_path = false;

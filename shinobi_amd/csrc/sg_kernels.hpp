@@ -55,29 +55,37 @@ struct GridView {
     const uint32_t* leaves;    // two-level: 2*C2 ranges
     const uint32_t* indices;   // reference -> triangle id
     const float4* rec;         // per-triangle record (4 x float4, see TriSoA)
-    const float4* refs;        // per-reference record (3 x float4, see launch_ref_records)
+    const float* refs;         // per-reference records (9 floats each, see write_ref_record)
     int kind;                  // 0 uniform, 1 two-level
 };
 
 // ---------------------------------------------------------------- build
 // Reference records: for every reference i < *d_valid of the final grid,
-// refs[3i..3i+2] = {v0, e1.x}, {e1.yz, e2.xy}, {e2.z, triangle id, 0, 0} of
-// triangle indices[i] -- the 36 bytes a triangle test reads, in the order
-// the traversal reads them (no index indirection, one 48-byte record).
-SG_D void write_ref_record(float4* __restrict__ refs, uint32_t i, const float4* __restrict__ rec, uint32_t tri,
+// refs[9i..9i+8] = v0, e1 = v1 - v0, e2 = v2 - v0 of triangle indices[i] --
+// the 36 bytes a triangle test reads, in the order the traversal reads them
+// (no index indirection; the triangle id comes from indices[] once per ray).
+// Records are packed at a 36-byte stride and read as two 16-byte loads at
+// 4-byte alignment plus one word: 25 % fewer bytes than 48-byte aligned
+// records, which measured faster on gfx950 (more of them stay in the
+// memory-side cache).
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+
+SG_D void write_ref_record(float* __restrict__ refs, uint32_t i, const float4* __restrict__ rec, uint32_t tri,
                            uint32_t n_tris) {
-    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+    float c = 0.f;
     if (tri < n_tris) {   // always, unless the build overflowed (never traversed then)
         a = rec[4 * (size_t)tri];
         b = rec[4 * (size_t)tri + 1];
-        c = make_float4(rec[4 * (size_t)tri + 2].x, __uint_as_float(tri), 0.f, 0.f);
+        c = rec[4 * (size_t)tri + 2].x;
     }
-    refs[3 * (size_t)i] = a;
-    refs[3 * (size_t)i + 1] = b;
-    refs[3 * (size_t)i + 2] = c;
+    float* o = refs + 9 * (size_t)i;
+    *(f4u*)o = f4u{a.x, a.y, a.z, a.w};
+    *(f4u*)(o + 4) = f4u{b.x, b.y, b.z, b.w};
+    o[8] = c;
 }
 void launch_ref_records(const uint32_t* indices, const uint32_t* d_valid, const uint32_t* d_gate, uint32_t cap,
-                        const float4* rec, uint32_t n_tris, float4* refs, hipStream_t st);
+                        const float4* rec, uint32_t n_tris, float* refs, hipStream_t st);
 void launch_init(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, hipStream_t st);
 // per-block bounds partials (6 words per block, bounds_blocks(n) blocks) and
 // the triangle records
@@ -112,7 +120,7 @@ struct LeafLarge {
 // refs: reference records of the accepted pairs of LDS-sorted tiles
 void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const uint32_t* top_keys,
                       const uint32_t* top_vals, const uint32_t* ranges, const uint32_t* cells, uint32_t* keys,
-                      uint32_t* vals, float4* refs, uint32_t* dbg, uint32_t* sums, uint32_t* info,
+                      uint32_t* vals, float* refs, uint32_t* dbg, uint32_t* sums, uint32_t* info,
                       const LeafLarge* large, hipStream_t st);
 // global stable sort of the leaf pairs of tiles too large for LDS (no work
 // when there are none); the result stays in (k0, v0)

@@ -4,12 +4,12 @@
 // One ray per lane.  Rays are SoA (coalesced loads), grid cells / leaves
 // are plain buffers (the reference's texture fetches replaced by explicit
 // clamp-to-edge and zero-out-of-range reads), and reference i of a cell is
-// tested from its 48-byte reference record (GridView::refs, written by the
-// build in reference order): {v0, e1 = v1-v0, e2 = v2-v0, triangle id}.  The
-// edge subtractions are the binary32 operations the intersector would
-// perform, so results are unchanged; a test is two 16-byte loads and one
-// 4-byte load at an address known from the cell range alone (no index
-// fetch between the range and the triangle).
+// tested from its 36-byte reference record (GridView::refs, written by the
+// build in reference order): {v0, e1 = v1-v0, e2 = v2-v0}.  The edge
+// subtractions are the binary32 operations the intersector would perform,
+// so results are unchanged; a test is two 16-byte loads and one 4-byte load
+// at an address known from the cell range alone (no index fetch between the
+// range and the triangle).
 #include <algorithm>
 
 #include "sg_device.hpp"
@@ -42,7 +42,7 @@ using gptr = __attribute__((address_space(1))) T*;
 struct GView {
     gptr<const GridHeader> hdr;
     gptr<const uint32_t> cells, leaves, indices;
-    gptr<const float4> refs;
+    gptr<const float> refs;
 };
 
 struct GRays {
@@ -57,6 +57,10 @@ SG_D float4 ld4(gptr<const float4> p) {
     const f4v v = *(gptr<const f4v>)p;
     return make_float4(v.x, v.y, v.z, v.w);
 }
+SG_D float4 ld4u(gptr<const float> p) {   // 16 bytes at 4-byte alignment
+    const f4u v = *(gptr<const f4u>)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 SG_D uint2 ld2(gptr<const uint32_t> p) {
     const u2v v = *(gptr<const u2v>)p;
     return make_uint2(v.x, v.y);
@@ -68,7 +72,7 @@ SG_D GView gview(const GridView& g) {
     v.cells = (gptr<const uint32_t>)g.cells;
     v.leaves = (gptr<const uint32_t>)g.leaves;
     v.indices = (gptr<const uint32_t>)g.indices;
-    v.refs = (gptr<const float4>)g.refs;
+    v.refs = (gptr<const float>)g.refs;
     return v;
 }
 
@@ -121,7 +125,7 @@ constexpr uint32_t kTopWaitMax = 3; // iterations a lane waits for it at most
 // (t, reference), which is what the ordered loop with strict < returns (the
 // smallest distance, the first reference among equal ones).  Returns the
 // key (distance bits << 32 | reference), ~0 when nothing closer than t hit.
-__device__ __attribute__((noinline)) uint64_t coop_tests(gptr<const float4> refs, uint32_t lb, uint32_t le,
+__device__ __attribute__((noinline)) uint64_t coop_tests(gptr<const float> refs, uint32_t lb, uint32_t le,
                                                          float ox, float oy, float oz, float Dx, float Dy, float Dz,
                                                          float t0, uint64_t act) {
     const uint32_t nact = (uint32_t)__popcll(act);
@@ -132,8 +136,8 @@ __device__ __attribute__((noinline)) uint64_t coop_tests(gptr<const float4> refs
     r.t = t0;
     r.best = kMiss;
     for (uint32_t it = lb + rank; it < le; it += nact) {
-        const gptr<const float4> q = refs + 3 * (size_t)it;
-        mt_test(ld4(q), ld4(q + 1), *(gptr<const float>)(q + 2), it, r);
+        const gptr<const float> q = refs + 9 * (size_t)it;
+        mt_test(ld4u(q), ld4u(q + 4), q[8], it, r);
     }
     // positive distances order as unsigned integers: the smallest key is the
     // smallest t, then the first reference
@@ -334,10 +338,10 @@ struct RefRec {
 };
 
 SG_D void ref_load(const GView& g, const Refs& q, RefRec& x) {
-    const gptr<const float4> rr = g.refs + 3 * (size_t)q.it;
-    x.a = ld4(rr);
-    x.b = ld4(rr + 1);
-    x.c = *(gptr<const float>)(rr + 2);
+    const gptr<const float> rr = g.refs + 9 * (size_t)q.it;
+    x.a = ld4u(rr);
+    x.b = ld4u(rr + 4);
+    x.c = rr[8];
 }
 
 // min of the three exit distances: r.t > tmin(m) is r.t > m.x || r.t > m.y
