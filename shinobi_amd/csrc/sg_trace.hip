@@ -51,12 +51,7 @@ struct GRays {
 
 // 16- and 8-byte loads through global pointers (the HIP vector types cannot
 // be copied from a global-typed reference, clang's vector types can)
-typedef float f4v __attribute__((ext_vector_type(4)));
 typedef uint32_t u2v __attribute__((ext_vector_type(2)));
-SG_D float4 ld4(gptr<const float4> p) {
-    const f4v v = *(gptr<const f4v>)p;
-    return make_float4(v.x, v.y, v.z, v.w);
-}
 SG_D float4 ld4u(gptr<const float> p) {   // 16 bytes at 4-byte alignment
     const f4u v = *(gptr<const f4u>)p;
     return make_float4(v.x, v.y, v.z, v.w);
