@@ -120,7 +120,7 @@ constexpr uint32_t kTopWaitMax = 3; // iterations a lane waits for it at most
 // (t, reference), which is what the ordered loop with strict < returns (the
 // smallest distance, the first reference among equal ones).  Returns the
 // key (distance bits << 32 | reference), ~0 when nothing closer than t hit.
-__device__ __attribute__((noinline)) uint64_t coop_tests(gptr<const float> refs, uint32_t lb, uint32_t le,
+__device__ __attribute__((always_inline)) uint64_t coop_tests(gptr<const float> refs, uint32_t lb, uint32_t le,
                                                          float ox, float oy, float oz, float Dx, float Dy, float Dz,
                                                          float t0, uint64_t act) {
     const uint32_t nact = (uint32_t)__popcll(act);
@@ -576,9 +576,18 @@ SG_D void trace_one(const GView& g, const GRays& rays, uint32_t i, gptr<uint32_t
     }
 }
 
+// Occupancy of the trace kernels: the traversal is bound by memory latency
+// and issue, and waves per SIMD measured 3 / 4 / 5 -> 1.56 / 1.80 / 1.99
+// Grays/s on the height field.  The two-level walk fits 6 waves in 80 VGPRs
+// with a 12-byte spill (per-ray constants reloaded where a leaf walk
+// starts), 1-2 % faster than 5 waves without; at 7 waves the spills reach
+// the test path and it runs 15 % slower.  coop_tests is inlined: as a call
+// it held 12 more VGPRs across the call.
+#define SG_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(6)))
+
 // one ray per lane over one batch; the block size is a launch parameter
 template <int KIND, bool STATS, bool SHADOW>
-__global__ __launch_bounds__(kBlock) void k_trace(GridView g, RaySoA rays, uint32_t cap, const uint32_t* d_n,
+__global__ __launch_bounds__(kBlock) SG_TRACE_OCCUPANCY void k_trace(GridView g, RaySoA rays, uint32_t cap, const uint32_t* d_n,
                                                   uint32_t* __restrict__ prim, float* __restrict__ tout,
                                                   uint32_t* __restrict__ stats) {
     const uint32_t n = d_n ? min(*d_n, cap) : cap;
@@ -618,7 +627,7 @@ void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t
 // kind: a kernel that could run either walk would hold the registers of
 // the larger one.
 template <int KIND, bool SHADOW>
-__global__ __launch_bounds__(kBlock) void k_trace_items(const TraceItem* __restrict__ items, uint32_t n_items,
+__global__ __launch_bounds__(kBlock) SG_TRACE_OCCUPANCY void k_trace_items(const TraceItem* __restrict__ items, uint32_t n_items,
                                                         uint32_t total_chunks) {
     const uint32_t c = blockIdx.x * (kBlock / 64u) + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (c >= total_chunks) return;
