@@ -8,7 +8,7 @@ Workload (one step = one pass of the hot path over one batch):
             resolution-fixed top level lambda 5 / 6, leaf lambda 2, every
             phase on the device, no host sync)
          2. closest-hit trace of 1920x1080 primary rays from (0.5,-0.3,0.6),
-            ordered in 8x8 pixel tiles (the row-major time is reported as
+            ordered in 2x32 pixel tiles (the row-major time is reported as
             trace_primary_rowmajor_ms)
          3. one cosine-weighted diffuse bounce per hit (device generator,
             compacted and grouped by direction class)
@@ -36,7 +36,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 FP32_PEAK_TFLOPS = 157.3       # MI355X vector FP32, TFLOP/s
 
-PRIMARY_TILE = 8   # primary rays ordered in 8x8 pixel tiles (workloads.tile_order)
+# Primary rays are traced in 2x32-pixel column tiles (workloads.tile_order):
+# a wave then holds near and far rays, and the lanes that finish early test
+# the remaining rays' cells with them (wave-shared tests).  Measured on the
+# height field: 2x32 0.51 ms, 4x16 0.56, 8x8 0.59, row-major 0.62; on the
+# ruins camera 2x32 0.55, 8x8 0.60, row-major 0.70.
+PRIMARY_TILE = (2, 32)
 
 PHASES = ["bounds", "top_emit", "top_sort", "top_ranges", "leaf_cells", "leaf_emit", "leaf_sort", "leaf_ranges"]
 
@@ -231,7 +236,7 @@ def ruins_bench(sa, W, ctx, reps: int = 10) -> dict:
     1024x1024 camera.cfg view): two-level rebuild with the reference
     builder's defaults, primary trace, one diffuse bounce, bounce trace."""
     golden = os.path.join(ROOT, "tests", "golden")
-    w = W.scene_file(os.path.join(golden, "ruins.obj.gz"), os.path.join(golden, "ruins_camera.cfg"))
+    w = W.scene_file(os.path.join(golden, "ruins.obj.gz"), os.path.join(golden, "ruins_camera.cfg"), tile=PRIMARY_TILE)
     scene = sa.FaceSoup(ctx, w.scene)
     b = sa.FastTwoLevelGridBuilder(ctx, **w.build)
     b.init(scene)
@@ -515,7 +520,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "heightfield2m: 2,093,058-tri height field, two-level grid "
                                "(lambda 5 / 6 top, lambda 2 leaf), 1920x1080 primary + 1 diffuse bounce per hit",
-                   "primary_order": f"{PRIMARY_TILE}x{PRIMARY_TILE} pixel tiles",
+                   "primary_order": f"{PRIMARY_TILE[0]}x{PRIMARY_TILE[1]} pixel tiles",
                    "n_tris": n_tris, "primary_rays": n_rays, "bounce_rays": m, "res_top": list(info.res),
                    "C1": info.C1, "P1": info.P1, "C2": info.C2, "P2": info.P2},
         "grid_bytes": info.device_bytes,

@@ -56,25 +56,27 @@ def heightfield_camera(seed: int, res_x: int, res_y: int, amplitude: float = 0.2
     return gen_camera_rays(eye, fwd, (0.0, 0.0, 1.0), 60.0, res_x, res_y)
 
 
-def tile_order(res_x: int, res_y: int, tile: int) -> np.ndarray:
-    """Row-major pixel ids visited tile by tile: tile x tile blocks in
-    row-major order, row-major inside a block (partial blocks at the right
-    and bottom edges keep their pixels).  tile = 1 is the identity."""
+def tile_order(res_x: int, res_y: int, tile) -> np.ndarray:
+    """Row-major pixel ids visited tile by tile: tiles of tile = w (square)
+    or (w, h) pixels in row-major order, row-major inside a tile (partial
+    tiles at the right and bottom edges keep their pixels).  tile = 1 is the
+    identity."""
+    tw, th = (tile, tile) if np.isscalar(tile) else tile
     ids = np.arange(res_x * res_y, dtype=np.int64).reshape(res_y, res_x)
-    if tile <= 1:
+    if tw <= 1 and th <= 1:
         return ids.reshape(-1)
-    out = [ids[y0:y0 + tile, x0:x0 + tile].reshape(-1)
-           for y0 in range(0, res_y, tile) for x0 in range(0, res_x, tile)]
+    out = [ids[y0:y0 + th, x0:x0 + tw].reshape(-1)
+           for y0 in range(0, res_y, th) for x0 in range(0, res_x, tw)]
     return np.concatenate(out)
 
 
 def heightfield2m(grid: int = 1024, res_x: int = 1920, res_y: int = 1080, seed: int = 2,
-                  tile: int = 1) -> Workload:
-    """tile > 1: the same primary rays ordered tile by tile (tile_order), the
-    pixel order a renderer uses to keep the rays of a wave close together"""
+                  tile=1) -> Workload:
+    """tile (w or (w, h)) other than 1: the same primary rays ordered tile by
+    tile (tile_order), a renderer's choice of which pixels share a wave"""
     scene = gen_heightfield(seed, grid, 0.2)
     rays = heightfield_camera(seed, res_x, res_y)
-    if tile > 1:
+    if tile != 1:
         rays = np.ascontiguousarray(rays[:, tile_order(res_x, res_y, tile)])
     return Workload("heightfield2m", SG_GRID_TWO_LEVEL, scene, rays,
                     dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16), bounce_seed=seed + 100,
@@ -116,8 +118,9 @@ def sphere_mesh(n_tris: int = 200_000, frame: int = 0, seed: int = 5) -> np.ndar
     return np.ascontiguousarray(np.concatenate([t1, t2], axis=1), np.float32)
 
 
-def scene_file(obj_path: str, camera_path: str, name: str = "ruins") -> Workload:
-    """An OBJ scene (plain or .gz) with the primary rays of its camera.cfg."""
+def scene_file(obj_path: str, camera_path: str, name: str = "ruins", tile=1) -> Workload:
+    """An OBJ scene (plain or .gz) with the primary rays of its camera.cfg
+    (row-major, or tile by tile as tile_order for tile other than 1)."""
     if obj_path.endswith(".gz"):
         with tempfile.TemporaryDirectory() as d:
             plain = os.path.join(d, "scene.obj")
@@ -128,6 +131,8 @@ def scene_file(obj_path: str, camera_path: str, name: str = "ruins") -> Workload
         scene, _ = load_obj(obj_path)
     cam = load_camera(camera_path)
     rays = gen_camera_rays(cam["position"], cam["orientation"], cam["up"], cam["fov_deg"], cam["res_x"], cam["res_y"])
+    if tile != 1:
+        rays = np.ascontiguousarray(rays[:, tile_order(cam["res_x"], cam["res_y"], tile)])
     return Workload(name, SG_GRID_TWO_LEVEL, scene, rays,
                     dict(lam=5.0, top_divisor=6, lambda_leaf=1.2, leaf_min_refs=16), bounce_seed=107,
                     oracle=dict(lam=5.0, top_divisor=6, lambda_leaf=1.2, leaf_min_refs=16))
