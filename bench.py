@@ -11,7 +11,7 @@ Workload (one step = one pass of the hot path over one batch):
             ordered in 2x32 pixel tiles (the row-major time is reported as
             trace_primary_rowmajor_ms)
          3. one cosine-weighted diffuse bounce per hit (device generator,
-            compacted and grouped by direction class)
+            compacted and ordered by direction class in rounds of 256)
          4. closest-hit trace of the bounce rays (count read on the device)
 value  = closest-hit Mrays/s = (primary + bounce rays) / (time of the two
          trace kernels), measured with HIP events on the stream the kernels

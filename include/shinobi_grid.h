@@ -181,9 +181,10 @@ sg_status sg_trace_shadow(sg_ctx *ctx, sg_grid *grid, const sg_rays *rays, uint3
                           const uint32_t *d_count, uint32_t *d_prim, float *d_t, uint32_t *d_stats,
                           void *stream);
 /* one cosine-weighted diffuse bounce per hit (SimplePathTracer.hpp:115-166),
- * compacted and grouped by direction class (4 bins of each component of the
- * new direction, class = bx | by<<2 | bz<<4; input order within a class), the
- * order in which incoherent secondary rays trace fastest; *d_count = number
+ * compacted and ordered by direction class (4 bins of each component of the
+ * new direction, class = bx | by<<2 | bz<<4) in rounds of 256 rays: sorted by
+ * (r / 256, class, r % 256), r the ray's rank within its class in input
+ * order -- waves of one class, the classes spread over the launch; *d_count = number
  * of bounce rays; d_src (opt.) = input ray index of each bounce ray.
  * n = capacity of the input; d_n
  * (optional device pointer) = the number of input rays (<= n), so bounces

@@ -1083,7 +1083,8 @@ static uint32_t dir_bin(float c)
 uint32_t or_bounce(const float *const v[9], const float *const rays[7], const uint32_t *prim,
                    const float *t, uint32_t nrays, uint64_t seed, float *const out[7], uint32_t *src)
 {
-    /* the bounce rays in input order, then grouped stably by direction class */
+    /* the bounce rays in input order, then ordered by (r / 256, class, r % 256),
+     * r the ray's rank within its direction class in input order */
     float *tmp = (float *)malloc((size_t)7 * (nrays ? nrays : 1) * sizeof(float));
     uint32_t *tsrc = (uint32_t *)malloc((size_t)(nrays ? nrays : 1) * sizeof(uint32_t));
     uint32_t *cls = (uint32_t *)malloc((size_t)(nrays ? nrays : 1) * sizeof(uint32_t));
@@ -1114,11 +1115,19 @@ uint32_t or_bounce(const float *const v[9], const float *const rays[7], const ui
         cls[m] = dir_bin(nd.x) | dir_bin(nd.y) << 2 | dir_bin(nd.z) << 4;
         ++m;
     }
-    uint32_t start[65] = {0};
-    for (uint32_t k = 0; k < m; ++k) ++start[cls[k] + 1];
-    for (int c = 0; c < 64; ++c) start[c + 1] += start[c];
+    uint32_t tot[64] = {0}, seen[64] = {0};
+    for (uint32_t k = 0; k < m; ++k) ++tot[cls[k]];
     for (uint32_t k = 0; k < m; ++k) {
-        const uint32_t p = start[cls[k]]++;
+        const uint32_t c = cls[k], r = seen[c]++;
+        const uint32_t j0 = r / 256u * 256u;
+        uint32_t p = r - j0;   /* rays of earlier rounds, then this round's of earlier classes */
+        for (uint32_t q = 0; q < 64; ++q) {
+            p += tot[q] < j0 ? tot[q] : j0;
+            if (q < c) {
+                const uint32_t left = tot[q] > j0 ? tot[q] - j0 : 0u;
+                p += left < 256u ? left : 256u;
+            }
+        }
         for (int a = 0; a < 7; ++a) out[a][p] = tmp[(size_t)7 * k + a];
         src[p] = tsrc[k];
     }

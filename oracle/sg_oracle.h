@@ -128,9 +128,11 @@ void or_brute_f64(const float *const v[9], uint32_t n,
 
 /* ---- secondary rays --------------------------------------------------------
  * One cosine-weighted diffuse bounce per hit (SimplePathTracer.hpp:115-166),
- * compacted and grouped stably by direction class (bin floor(2 (c + 1)) of
- * each component of the new direction, clamped to 0..3; class = bx | by<<2 |
- * bz<<4), the order the library's generator writes.  Returns the number of
+ * compacted and ordered by direction class (bin floor(2 (c + 1)) of each
+ * component of the new direction, clamped to 0..3; class = bx | by<<2 |
+ * bz<<4) in rounds of 256: sorted by (r / 256, class, r % 256), r the ray's
+ * rank within its class in input order -- the order the library's generator
+ * writes.  Returns the number of
  * rays written; src[i] is the primary ray each bounce came from. */
 uint32_t or_bounce(const float *const v[9], const float *const rays[7],
                    const uint32_t *prim, const float *t, uint32_t nrays,

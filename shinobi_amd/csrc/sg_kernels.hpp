@@ -172,9 +172,9 @@ void launch_trace_items(const TraceItem* items, const ItemSplit& split, bool sha
 // (block: threads per block, 64, 128 or 256)
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim,
                   float* t, uint32_t* stats, bool shadow, int block, hipStream_t st);
-// one cosine-weighted bounce per hit, compacted and grouped by direction
-// class (input order within a class); *d_count receives the number of
-// bounce rays.  scratch: bounce_scratch_bytes(n).
+// one cosine-weighted bounce per hit, compacted and ordered by direction
+// class in rounds of 256 rays (input order within a class); *d_count
+// receives the number of bounce rays.  scratch: bounce_scratch_bytes(n).
 size_t bounce_scratch_bytes(uint32_t n);
 void launch_bounce(const TriSoA& tris, const RaySoA& r, const uint32_t* prim, const float* t, uint32_t n,
                    const uint32_t* d_n, uint64_t seed, const RaySoAOut& out, uint32_t* src, uint32_t* d_count,

@@ -115,18 +115,29 @@ def _dir_class(d):
     return bins[0] | bins[1] << 2 | bins[2] << 4
 
 
+def _bounce_order_ok(cls, src, round_=256):
+    """the output is sorted by (rank // round_, class, src), rank the ray's
+    position within its class in input (src) order"""
+    rank = np.empty(len(src), np.int64)
+    for c in np.unique(cls):
+        idx = np.flatnonzero(cls == c)
+        rank[idx[np.argsort(src[idx])]] = np.arange(len(idx))
+    key = np.stack([rank // round_, cls.astype(np.int64), src.astype(np.int64)])
+    ordered = np.lexsort(key[::-1])
+    return np.array_equal(ordered, np.arange(len(src)))
+
+
 def test_bounce_rays(oracle, scene):
-    """one cosine-weighted bounce per hit: every hit exactly once, grouped by
-    direction class and in primary order within a class, unit directions in
-    the hemisphere facing away from the incoming ray, origin 0.001 off the
-    surface"""
+    """one cosine-weighted bounce per hit: every hit exactly once, ordered by
+    direction class in rounds of 256 rays (primary order within a class),
+    unit directions in the hemisphere facing away from the incoming ray,
+    origin 0.001 off the surface"""
     r = sa.gen_random_rays(40, 5000, 0.0, 1.0)
     g = oracle.build_uniform(scene, top_divisor=1)
     prim, t = g.trace(r)
     b, src = oracle.bounce(scene, r, prim, t, 7)
     cls = _dir_class(b[3:6])
-    assert (np.diff(cls) >= 0).all()
-    assert all((np.diff(src[cls == c].astype(np.int64)) > 0).all() for c in np.unique(cls))
+    assert _bounce_order_ok(cls, src)
     assert len(np.unique(cls)) > 16
     order = np.argsort(src)
     b, src = b[:, order], src[order]
@@ -145,6 +156,30 @@ def test_bounce_rays(oracle, scene):
     assert np.allclose(off, 0.001, atol=2e-5)
     # cosine weighting: E[cos theta] = 2/3
     assert abs(np.mean(np.sum(n * d, axis=0)) - 2.0 / 3.0) < 0.03
+
+
+def test_bounce_order_in_rounds(oracle, scene):
+    """classes with more than 256 bounce rays: each class's rays come in
+    rounds of 256 (sorted by round, class, input order), so a large class is
+    not contiguous and the classes alternate"""
+    r = sa.gen_random_rays(41, 60_000, 0.0, 1.0)
+    g = oracle.build_uniform(scene, top_divisor=1)
+    prim, t = g.trace(r)
+    b, src = oracle.bounce(scene, r, prim, t, 9)
+    cls = _dir_class(b[3:6])
+    assert _bounce_order_ok(cls, src)
+    cnt = np.bincount(cls, minlength=64)
+    big = int(np.argmax(cnt))
+    assert cnt[big] > 512
+    at = np.flatnonzero(cls == big)
+    assert at[256] - at[255] > 1 and at[255] - at[0] == 255
+    # the first round holds min(count, 256) rays of every class, in class order
+    n0 = int(np.minimum(cnt, 256).sum())
+    assert (np.diff(cls[:n0]) >= 0).all()
+    assert np.array_equal(np.bincount(cls[:n0], minlength=64), np.minimum(cnt, 256))
+    # a plausible mistake -- plain grouping by class -- fails the check
+    grouped = np.lexsort((src, cls))
+    assert not _bounce_order_ok(cls[grouped], src[grouped])
 
 
 def _shadow_rays(oracle, g, scene, rays, light):
