@@ -480,7 +480,7 @@ def main() -> int:
     phase_gbs = {p: round(bb[p] / (mean(ph[p]) * 1e-3) / 1e9, 1) for p in PHASES}
     build_gbs = total_bytes / (mean(build_ms) * 1e-3) / 1e9
 
-    # roofline of the dominant kernel: k_trace<1,false,false>, the two-level
+    # roofline of the dominant kernel: k_trace<1,false,false,false>, the two-level
     # closest-hit traversal, launched twice per step (primary, bounce); the
     # object describes its longer launch, both launches are listed beside it
     launches = {}
@@ -491,7 +491,7 @@ def main() -> int:
     dom = max(launches, key=lambda k: launches[k]["ms"])
     dominant = max([("trace_primary", mean(tp_ms)), ("trace_bounce", mean(tb_ms))] +
                    [(p, mean(v)) for p, v in ph.items()], key=lambda x: x[1])
-    roofline = {"kernel": f"k_trace<1,false,false> (two-level closest hit, {dom} rays)", "bound": "valu",
+    roofline = {"kernel": f"k_trace<1,false,false,false> (two-level closest hit, {dom} rays)", "bound": "valu",
                 "achieved": launches[dom]["achieved"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": launches[dom]["frac"], "traffic": launches[dom]["traffic"],
                 "flops_per_launch": launches[dom]["flops"], "launches": launches,

@@ -536,25 +536,30 @@ SG_D void walk_twolevel(const GView& g, const GParams& p, Ray& r, Walk& w, uint3
 
 // the trace kernel of src/Engine/RayTracingKernels.h:37-136 for one SoA ray:
 // rcp(dir) (:111-113), traverse, store prim = indices[best] (miss: ~0u) and t
-template <int KIND, bool STATS, bool SHADOW>
+// SPARSE: lanes without a ray (valid false) run the walk loop inactive, so
+// that they take part in the wave-shared tests; otherwise every lane has one.
+template <int KIND, bool STATS, bool SHADOW, bool SPARSE = false>
 SG_D void trace_one(const GView& g, const GRays& rays, uint32_t i, gptr<uint32_t> prim, gptr<float> tout,
-                    gptr<uint32_t> stats) {
+                    gptr<uint32_t> stats, bool valid = true) {
     // a failed or overflowed build is never traversed: every ray misses
     if (g.hdr->status & ~(ST_LEAF_RES | ST_NEG_EXTENT)) {
-        prim[i] = kMiss;
-        tout[i] = rays.a[6][i];
-        if (STATS)
-            for (int k = 0; k < 4; ++k) stats[4 * (size_t)i + k] = 0;
+        if (!SPARSE || valid) {
+            prim[i] = kMiss;
+            tout[i] = rays.a[6][i];
+            if (STATS)
+                for (int k = 0; k < 4; ++k) stats[4 * (size_t)i + k] = 0;
+        }
         return;
     }
     const GParams p = load_gparams(g.hdr);
+    const uint32_t j = SPARSE && !valid ? 0u : i;   // a lane without a ray reads ray 0, inactive
     Ray r;
-    r.o = mk(rays.a[0][i], rays.a[1][i], rays.a[2][i]);
-    const v3 d = mk(rays.a[3][i], rays.a[4][i], rays.a[5][i]);
+    r.o = mk(rays.a[0][j], rays.a[1][j], rays.a[2][j]);
+    const v3 d = mk(rays.a[3][j], rays.a[4][j], rays.a[5][j]);
     r.rcp = mk(rcp_rn(d.x), rcp_rn(d.y), rcp_rn(d.z));
     r.D = mk(rcp_rn(r.rcp.x), rcp_rn(r.rcp.y), rcp_rn(r.rcp.z));
     r.fast = div_fast_range(r.rcp.x) && div_fast_range(r.rcp.y) && div_fast_range(r.rcp.z);
-    r.t = rays.a[6][i];
+    r.t = SPARSE && !valid ? 0.f : rays.a[6][j];   // t = 0: never traversed
     r.best = kMiss;
     r.tests = 0;
     uint32_t s1 = 0, s2 = 0, s3 = 0;
@@ -566,6 +571,7 @@ SG_D void trace_one(const GView& g, const GRays& rays, uint32_t i, gptr<uint32_t
         walk_init_twolevel<SHADOW>(p, r, w);
         walk_twolevel<SHADOW>(g, p, r, w, s1, s2, s3);
     }
+    if (SPARSE && !valid) return;
     prim[i] = r.best == kMiss ? kMiss : g.indices[r.best];
     tout[i] = r.t;
     if (STATS) {
@@ -585,16 +591,55 @@ SG_D void trace_one(const GView& g, const GRays& rays, uint32_t i, gptr<uint32_t
 // it held 12 more VGPRs across the call.
 #define SG_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(6)))
 
-// one ray per lane over one batch; the block size is a launch parameter
-template <int KIND, bool STATS, bool SHADOW>
+// Rays per wave for a batch of n rays: 64, or fewer (down to 8) while the
+// chip (`fill` waves at once) would not be full.  A small batch -- a late
+// path segment, a small request -- lasts as long as its longest ray, and
+// the lanes without a ray then help the others through the wave-shared
+// tests.  The sparse form needs at most `fill` waves.
+SG_HD uint32_t rays_per_wave(uint32_t n, uint32_t fill) {
+    uint32_t R = 64;
+    while (R > 8u && (uint64_t)n <= (uint64_t)(R / 2u) * fill) R /= 2u;
+    return R;
+}
+
+// One batch, the block size a launch parameter.  SPARSE = false: one ray
+// per lane, for batches of 64 rays per wave; SPARSE = true: rays_per_wave
+// rays per wave.  With a device-side count both are launched and each
+// exits unless the count is its case, so the dense kernel carries none of
+// the sparse form's registers.
+template <int KIND, bool STATS, bool SHADOW, bool SPARSE>
 __global__ __launch_bounds__(kBlock) SG_TRACE_OCCUPANCY void k_trace(GridView g, RaySoA rays, uint32_t cap, const uint32_t* d_n,
-                                                  uint32_t* __restrict__ prim, float* __restrict__ tout,
-                                                  uint32_t* __restrict__ stats) {
+                                                  uint32_t fill, uint32_t* __restrict__ prim,
+                                                  float* __restrict__ tout, uint32_t* __restrict__ stats) {
     const uint32_t n = d_n ? min(*d_n, cap) : cap;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    trace_one<KIND, STATS, SHADOW>(gview(g), grays(rays), i, (gptr<uint32_t>)prim, (gptr<float>)tout,
-                                   (gptr<uint32_t>)stats);
+    const uint32_t R = rays_per_wave(n, fill);
+    if (SPARSE != (R < 64u)) return;
+    if (!SPARSE) {
+        const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= n) return;
+        trace_one<KIND, STATS, SHADOW>(gview(g), grays(rays), i, (gptr<uint32_t>)prim, (gptr<float>)tout,
+                                       (gptr<uint32_t>)stats);
+    } else {
+        const uint32_t wv = blockIdx.x * (blockDim.x / 64u) + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        if (wv * R >= n) return;   // the whole wave has no ray
+        const uint32_t l = (uint32_t)lane_id(), i = wv * R + l;
+        trace_one<KIND, STATS, SHADOW, true>(gview(g), grays(rays), i, (gptr<uint32_t>)prim, (gptr<float>)tout,
+                                             (gptr<uint32_t>)stats, l < R && i < n);
+    }
+}
+
+// waves the device holds at once (6 per SIMD, 4 SIMDs per CU), per device
+static uint32_t device_fill() {
+    static uint32_t cache[64] = {0};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= 64) return 6144u;
+    if (!cache[dev]) {
+        int cus = 0;
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        cache[dev] = (uint32_t)(cus > 0 ? cus : 256) * 24u;
+    }
+    return cache[dev];
 }
 
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim, float* t,
@@ -603,8 +648,20 @@ void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t
     // one ray per lane keeps the 64 lanes of a wave on neighbouring rays, so
     // coherent rays share cell, leaf and triangle loads
     const int B = (block == 64 || block == 128 || block == 256) ? block : kBlock;
-    const dim3 grid((n + B - 1) / B), tb(B);
-#define SG_TRACE(K, S, H) hipLaunchKernelGGL((k_trace<K, S, H>), grid, tb, 0, st, g, r, n, d_n, prim, t, stats)
+    const uint32_t fill = device_fill();
+    // dense grid: one lane per ray of the capacity; sparse grid: `fill` waves
+    // cover every count that takes the sparse form (n <= 32 fill, >= 8 rays
+    // a wave)
+    const dim3 dense((n + B - 1) / B), sparse((fill + B / 64 - 1) / (B / 64)), tb(B);
+    const bool want_dense = d_n || rays_per_wave(n, fill) == 64u;
+    const bool want_sparse = d_n || rays_per_wave(n, fill) < 64u;
+#define SG_TRACE(K, S, H)                                                                                         \
+    do {                                                                                                          \
+        if (want_dense)                                                                                           \
+            hipLaunchKernelGGL((k_trace<K, S, H, false>), dense, tb, 0, st, g, r, n, d_n, fill, prim, t, stats); \
+        if (want_sparse)                                                                                          \
+            hipLaunchKernelGGL((k_trace<K, S, H, true>), sparse, tb, 0, st, g, r, n, d_n, fill, prim, t, stats); \
+    } while (0)
     if (g.kind == 0) {
         if (shadow) { if (stats) SG_TRACE(0, true, true); else SG_TRACE(0, false, true); }
         else { if (stats) SG_TRACE(0, true, false); else SG_TRACE(0, false, false); }
