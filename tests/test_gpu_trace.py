@@ -223,3 +223,37 @@ def test_cluster_paths_depth4(ctx, oracle):
             rays_o, _ = oracle.bounce(w.scene, rays_o, po, to, w.bounce_seed + d)
             if rays_o.shape[1] == 0:
                 break
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("shadow", [False, True])
+@pytest.mark.parametrize("count", [1, 63, 777, 5_000, 40_000, 150_000, 250_000])
+def test_device_count_small_batches(ctx, oracle, kind, shadow, count):
+    """a device-side ray count far below the capacity: the batch is traced
+    with fewer rays per wave (8 to 32; 64 for the largest), lanes without a
+    ray taking part in the wave-shared tests; ids, t and the untouched
+    outputs beyond the count are checked"""
+    scene = sa.gen_soup(90 + kind, 20_000, 0.02)
+    cap = 400_000
+    r7 = sa.gen_random_rays(91, cap, 0.0, 1.0)
+    if shadow:
+        r7[6] = 1.0
+    soup = sa.FaceSoup(ctx, scene)
+    B = sa.FastTwoLevelGridBuilder if kind else sa.FastSimpleGridBuilder
+    b = B(ctx, lam=5.0) if kind == 0 else B(ctx, lam=5.0, lambda_leaf=2.0)
+    b.init(soup)
+    g = b.build()
+    rb = sa.RayBatch(ctx, r7)
+    d_n = sa.DeviceBuffer(ctx, 4)
+    d_n.upload(np.array([count], np.uint32))
+    prim = sa.DeviceBuffer(ctx, 4 * cap)
+    t = sa.DeviceBuffer(ctx, 4 * cap)
+    prim.upload(np.full(cap, 0xABCDEF01, np.uint32))
+    g.trace(rb, prim=prim, t=t, d_count=d_n.ptr, n=cap, shadow=shadow)
+    og = oracle.build_twolevel(scene, lam=5.0, lambda_leaf=2.0) if kind else oracle.build_uniform(scene, lam=5.0,
+                                                                                                   top_divisor=1)
+    po, to = og.trace(np.ascontiguousarray(r7[:, :count]), shadow=shadow)
+    pg = prim.download(np.uint32, cap)
+    np.testing.assert_array_equal(pg[:count], po)
+    np.testing.assert_array_equal(t.download(np.float32, count).view(np.uint32), to.view(np.uint32))
+    assert (pg[count:count + 4096] == 0xABCDEF01).all()
