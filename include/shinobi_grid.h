@@ -65,7 +65,7 @@ typedef struct {
     float capacity_slack;     /* buffer headroom for rebuilds (0 = 1.25)       */
     int keep_debug;           /* keep per-triangle / per-pair counts           */
     int trace_block;          /* threads per block of this grid's traversal
-                                 launches: 64, 128 or 256 (0 = 256)           */
+                                 launches: 64, 128 or 256 (0 = 64)            */
 } sg_build_params;
 
 void sg_build_params_default(sg_build_params *p, int kind);

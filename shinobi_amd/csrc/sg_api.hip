@@ -57,7 +57,7 @@ void sg_build_params_default(sg_build_params* p, int kind) {
     p->leaf_min_refs = 16;
     p->radix_bits = 8;
     p->capacity_slack = 1.25f;
-    p->trace_block = 256;
+    p->trace_block = 64;
 }
 
 sg_status sg_device_count(int* out) {
