@@ -689,7 +689,7 @@ void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t
 template <int KIND, bool SHADOW>
 __global__ __launch_bounds__(kBlock) SG_TRACE_OCCUPANCY void k_trace_items(const TraceItem* __restrict__ items, uint32_t n_items,
                                                         uint32_t total_chunks) {
-    const uint32_t c = blockIdx.x * (kBlock / 64u) + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t c = blockIdx.x * (blockDim.x / 64u) + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (c >= total_chunks) return;
     // Last item whose first chunk is <= c.  The table is read with
     // readfirstlane, so the compiler knows every value is wave-uniform and
@@ -730,13 +730,16 @@ ItemSplit split_items(const TraceItem* in, uint32_t n, TraceItem* out) {
     return s;
 }
 
+constexpr uint32_t kItemWaves = 1;   // waves per block of the multi-batch launch
+
 void launch_trace_items(const TraceItem* items, const ItemSplit& split, bool shadow, hipStream_t st) {
     for (int k = 0; k < 2; ++k) {
         if (split.n[k] == 0 || split.chunks[k] == 0) continue;
         const TraceItem* part = items + (k ? split.n[0] : 0);
-        const uint32_t blocks = (split.chunks[k] + kBlock / 64 - 1) / (kBlock / 64);
+        const uint32_t blocks = (split.chunks[k] + kItemWaves - 1) / kItemWaves;
 #define SG_ITEMS(K, H) \
-    hipLaunchKernelGGL((k_trace_items<K, H>), dim3(blocks), dim3(kBlock), 0, st, part, split.n[k], split.chunks[k])
+    hipLaunchKernelGGL((k_trace_items<K, H>), dim3(blocks), dim3(64 * kItemWaves), 0, st, part, split.n[k], \
+                       split.chunks[k])
         if (k == 0) { if (shadow) SG_ITEMS(0, true); else SG_ITEMS(0, false); }
         else { if (shadow) SG_ITEMS(1, true); else SG_ITEMS(1, false); }
 #undef SG_ITEMS
