@@ -28,8 +28,10 @@ def test_fp64_referee(oracle, scene, rays, kind):
     g = (oracle.build_twolevel(scene, lambda_leaf=2.0) if kind else oracle.build_uniform(scene, top_divisor=1))
     prim, t = g.trace(rays)
     p64, t64 = oracle.brute_f64(scene, rays)
+    # the hit-id bar of the grid path (99.99 %): on these 6000 rays every
+    # id must match the referee
     agree = np.mean(prim == p64)
-    assert agree >= 0.995, agree
+    assert agree >= 0.9999, agree
     hit = (prim == p64) & (p64 != MISS)
     assert np.max(np.abs(t[hit] - t64[hit]) / t64[hit]) < 1e-4
 
