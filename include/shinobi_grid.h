@@ -194,6 +194,15 @@ sg_status sg_bounce(sg_ctx *ctx, sg_scene *scene, const sg_rays *rays, const uin
                     uint32_t *d_src, uint32_t *d_count, void *stream);
 
 /* ---------------------------------------------------------- primitives --- */
+/* Scratch of sg_bounce, sg_scan_u32 and sg_sort_pairs: the context keeps one
+ * device buffer per stream, so calls on different streams never share
+ * memory and calls on one stream are ordered by it.  A call that needs a
+ * larger buffer than its stream has allocates one; under graph capture it
+ * returns SG_ERR_INVALID_ARG instead (make the same call once on that
+ * stream before capturing).  An outgrown buffer stays allocated until
+ * sg_ctx_destroy, so a captured graph keeps valid memory; two replays of
+ * graphs captured on one stream share that stream's buffer and must not
+ * run at the same time. */
 /* ExclusiveScan / InclusiveScan (src/Utils/Scan.h:35-68) on device arrays */
 sg_status sg_scan_u32(sg_ctx *ctx, const uint32_t *d_in, uint32_t *d_out, uint32_t n, int inclusive,
                       void *stream);
@@ -236,8 +245,17 @@ sg_status sg_batch_add(sg_batch *batch, sg_grid *grid, const sg_rays *rays, uint
 /* upload the item table (stream-ordered; the host side may be changed after) */
 sg_status sg_batch_commit(sg_batch *batch, void *stream);
 /* trace every committed item, closest hit (shadow = 0) or any hit (1).
- * Enqueues kernels only, so it can be captured with sg_graph_begin/end. */
+ * Enqueues kernels only, so it can be captured with sg_graph_begin/end.
+ * Each launch also records, per item, the status bits of the item's grid
+ * as the traversal found them (a rebuild replayed in a graph that outgrew
+ * its buffers is traversed as all misses). */
 sg_status sg_batch_launch(sg_batch *batch, int shadow, void *stream);
+/* after the launch (or graph replay) on `stream` has run: waits for the
+ * stream and writes, for the first n committed items in sg_batch_add order,
+ * SG_OK or the status of a grid that was traversed as all misses
+ * (SG_ERR_CAPACITY for a rebuild that outgrew the grid's buffers).  No
+ * counterpart in the reference, whose builders exit on errors. */
+sg_status sg_batch_get_status(sg_batch *batch, void *stream, sg_status *out, uint32_t n);
 
 /* --------------------------------------------------------------- graphs --- */
 /* Stream capture into a replayable HIP graph: rebuilds (sg_rebuild),
@@ -290,6 +308,8 @@ typedef struct {
     double p50_us, p99_us, mean_us, max_us;   /* submit -> results in host memory */
     double wall_s;                            /* first submit -> last completion  */
     double mrays_per_s;                       /* rays / wall_s                    */
+    uint64_t regrown;                         /* dynamic rebuilds that outgrew their
+                                                 grid and were built again (not reset) */
 } sg_server_stats;
 typedef struct sg_server sg_server;
 sg_status sg_server_create(sg_ctx *ctx, const sg_server_config *cfg, sg_server **out);
@@ -303,7 +323,10 @@ sg_status sg_server_add_grid(sg_server *server, sg_grid *grid, uint32_t *scene_i
  * DynamicRTEngine::renderFrame, src/Engine/DynamicRTEngine.h:73-158, with
  * FastTwoLevelGridBuilder::rebuild at fixed resolution).  update_dynamic
  * returns once the new grid is in front: requests submitted after it see the
- * new mesh, requests admitted before it finish on the old one. */
+ * new mesh, requests admitted before it finish on the old one.  A rebuild
+ * that outgrows the back grid's buffers (a mesh deformed past the capacity
+ * slack) is built again with new buffers before the swap; a mesh that
+ * cannot be gridded at all returns its status and the front grid stays. */
 sg_status sg_server_add_dynamic(sg_server *server, const float *const soa9[9], uint32_t n,
                                 const sg_build_params *params, uint32_t *scene_id);
 sg_status sg_server_update_dynamic(sg_server *server, uint32_t scene_id, const float *const soa9[9]);
@@ -314,7 +337,11 @@ sg_status sg_server_update_dynamic(sg_server *server, uint32_t scene_id, const f
  * asynchronous.  SG_ERR_BUDGET if the request can never fit the budget. */
 sg_status sg_server_submit(sg_server *server, uint32_t scene_id, const float *const rays7[7], uint32_t n,
                            uint32_t *prim_out, float *t_out, uint64_t *request_id);
-/* wait for one request; latency_us (optional) = submit -> completion */
+/* wait for one request; latency_us (optional) = submit -> completion.
+ * Returns the request's status: SG_ERR_CAPACITY (or the grid's other
+ * failure status) when its grid was traversed as all misses because its
+ * last build failed or outgrew the buffers (for example a caller's rebuild
+ * of a grid added with sg_server_add_grid). */
 sg_status sg_server_wait(sg_server *server, uint64_t request_id, double *latency_us);
 /* wait for every submitted request */
 sg_status sg_server_drain(sg_server *server);

@@ -442,15 +442,16 @@ class FastTwoLevelGridBuilder(_Builder):
 
 def bounce(ctx: Context, scene: FaceSoup, rays: RayBatch, prim: DeviceBuffer, t: DeviceBuffer, seed: int,
            out: RayBatch, src: DeviceBuffer | None = None, count: DeviceBuffer | None = None,
-           d_n: C.c_void_p | None = None) -> DeviceBuffer:
+           d_n: C.c_void_p | None = None, stream=None) -> DeviceBuffer:
     """One cosine-weighted diffuse bounce per hit, compacted; returns the
     device counter holding the number of bounce rays.  d_n: optional device
-    count of the input rays (<= rays.n)."""
+    count of the input rays (<= rays.n); stream: defaults to the context's."""
     count = count or DeviceBuffer(ctx, 4)
     r = rays.struct()
     o = out.struct()
     check(ctx._l.sg_bounce(ctx.h, scene.h, C.byref(r), prim.ptr, t.ptr, rays.n, d_n, seed, C.byref(o),
-                           src.ptr if src is not None else None, count.ptr, ctx.stream), "sg_bounce")
+                           src.ptr if src is not None else None, count.ptr,
+                           ctx.stream if stream is None else stream), "sg_bounce")
     return count
 
 
@@ -526,6 +527,14 @@ class Batch:
         check(self.ctx._l.sg_batch_launch(self.h, int(shadow), self.ctx.stream if stream is None else stream),
               "sg_batch_launch")
 
+    def status(self, n: int, stream=None) -> list[int]:
+        """per item (sg_batch_add order), SG_OK or the status of a grid the
+        last launch traversed as all misses (sg_batch_get_status)"""
+        out = (C.c_int * max(n, 1))()
+        check(self.ctx._l.sg_batch_get_status(self.h, self.ctx.stream if stream is None else stream, out, n),
+              "sg_batch_get_status")
+        return [int(out[i]) for i in range(n)]
+
     def close(self) -> None:
         if getattr(self, "h", None):
             self.ctx._l.sg_batch_destroy(self.h)
@@ -598,6 +607,7 @@ class ServerStats:
     max_us: float
     wall_s: float
     mrays_per_s: float
+    regrown: int
 
 
 class Server:
@@ -655,8 +665,9 @@ class Server:
     def wait(self, rid: int):
         """-> (prim, t, latency_us)"""
         lat = C.c_double()
-        check(self.ctx._l.sg_server_wait(self.h, rid, C.byref(lat)), "sg_server_wait")
+        st = self.ctx._l.sg_server_wait(self.h, rid, C.byref(lat))
         _, prim, t = self._keep.pop(rid)
+        check(st, "sg_server_wait")
         return prim, t, float(lat.value)
 
     def drain(self) -> None:

@@ -96,7 +96,7 @@ class Camera(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("requests", C.c_uint64), ("rays", C.c_uint64), ("waves", C.c_uint64),
                 ("p50_us", C.c_double), ("p99_us", C.c_double), ("mean_us", C.c_double), ("max_us", C.c_double),
-                ("wall_s", C.c_double), ("mrays_per_s", C.c_double)]
+                ("wall_s", C.c_double), ("mrays_per_s", C.c_double), ("regrown", C.c_uint64)]
 
 
 # name -> (restype, argtypes); the authoritative list of exported entry points
@@ -146,6 +146,7 @@ SIGNATURES = {
     "sg_batch_add": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_uint32, C.c_void_p, C.c_void_p]),
     "sg_batch_commit": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sg_batch_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "sg_batch_get_status": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.c_uint32]),
     "sg_graph_begin": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sg_graph_end": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "sg_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -9,15 +9,6 @@
 
 #include "sg_internal.hpp"
 
-static sg_status status_from_bits(uint32_t b) {
-    if (b & ST_INVALID) return SG_ERR_INVALID_SCENE;
-    if (b & ST_TOO_MANY_CELLS) return SG_ERR_TOO_MANY_CELLS;
-    if (b & (ST_CAP_P1 | ST_CAP_C1 | ST_CAP_C2 | ST_CAP_P2)) return SG_ERR_CAPACITY;
-    if (b & ST_NEG_EXTENT) return SG_ERR_NEG_EXTENT;
-    if (b & ST_LEAF_RES) return SG_ERR_LEAF_RES;
-    return SG_OK;
-}
-
 static int bit_length(uint64_t x) {
     int b = 0;
     while (x) {
@@ -83,24 +74,31 @@ sg_status sg_ctx_create(int device, sg_ctx** out) {
 sg_status sg_ctx_destroy(sg_ctx* ctx) {
     if (!ctx) return SG_ERR_INVALID_ARG;
     hipSetDevice(ctx->device);
-    if (ctx->scratch) hipFree(ctx->scratch);
+    if (!ctx->scratch.empty() || !ctx->retired.empty()) hipDeviceSynchronize();
+    for (auto& kv : ctx->scratch) hipFree(kv.second.p);
+    for (void* p : ctx->retired) hipFree(p);
     delete ctx;
     return SG_OK;
 }
 
-static sg_status ctx_scratch(sg_ctx* ctx, size_t bytes, void** out) {
-    if (ctx->scratch_bytes < bytes) {
-        if (ctx->scratch) {
-            HIPCHK(hipDeviceSynchronize());
-            hipFree(ctx->scratch);
-            ctx->scratch = nullptr;
-            ctx->scratch_bytes = 0;
-        }
-        size_t b = std::max(bytes, (size_t)1 << 20);
-        HIPCHK(hipMalloc(&ctx->scratch, b));
-        ctx->scratch_bytes = b;
+// The scratch buffer of `stream` with at least `bytes` bytes.  Growing it
+// allocates, which a stream under graph capture must not do: such a call
+// returns SG_ERR_INVALID_ARG (make the same call once before capturing).
+static sg_status ctx_scratch(sg_ctx* ctx, hipStream_t stream, size_t bytes, void** out) {
+    std::lock_guard<std::mutex> lk(ctx->scratch_mu);
+    ScratchBuf& b = ctx->scratch[stream];
+    if (b.bytes < bytes) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIPCHK(hipStreamIsCapturing(stream, &cs));
+        if (cs != hipStreamCaptureStatusNone) return SG_ERR_INVALID_ARG;
+        const size_t nb = std::max(bytes, (size_t)1 << 20);
+        void* p = nullptr;
+        if (hipMalloc(&p, nb) != hipSuccess) return SG_ERR_OUT_OF_MEMORY;
+        if (b.p) ctx->retired.push_back(b.p);
+        b.p = p;
+        b.bytes = nb;
     }
-    *out = ctx->scratch;
+    *out = b.p;
     return SG_OK;
 }
 
@@ -586,7 +584,7 @@ sg_status sg_bounce(sg_ctx* ctx, sg_scene* scene, const sg_rays* rays, const uin
     HIPCHK(hipSetDevice(ctx->device));
     clear_last_error();
     void* scratch = nullptr;
-    sg_status r = ctx_scratch(ctx, bounce_scratch_bytes(n), &scratch);
+    sg_status r = ctx_scratch(ctx, (hipStream_t)stream, bounce_scratch_bytes(n), &scratch);
     if (r != SG_OK) return r;
     RaySoAOut o;
     for (int i = 0; i < 3; ++i) {
@@ -606,7 +604,7 @@ sg_status sg_scan_u32(sg_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint32
     HIPCHK(hipSetDevice(ctx->device));
     clear_last_error();
     void* scratch = nullptr;
-    sg_status r = ctx_scratch(ctx, scan_scratch_bytes(n), &scratch);
+    sg_status r = ctx_scratch(ctx, (hipStream_t)stream, scan_scratch_bytes(n), &scratch);
     if (r != SG_OK) return r;
     launch_scan(d_in, d_out, n, inclusive != 0, scratch, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
@@ -631,7 +629,7 @@ sg_status sg_sort_pairs(sg_ctx* ctx, uint32_t* d_keys, uint32_t* d_vals, uint32_
     const size_t sw = sort_scratch_words(n, key_bits, radix_bits);
     const size_t bytes = 256 + align_up((size_t)n * 4, 256) * 2 + sw * 4;
     void* scratch = nullptr;
-    sg_status r = ctx_scratch(ctx, bytes, &scratch);
+    sg_status r = ctx_scratch(ctx, st, bytes, &scratch);
     if (r != SG_OK) return r;
     uint32_t* words = (uint32_t*)scratch;
     uint32_t* tk = (uint32_t*)((char*)scratch + 256);

@@ -5,16 +5,31 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <vector>
+
 #include "../../include/shinobi_grid.h"
 #include "sg_kernels.hpp"
 
 using namespace sg;
 
+// Scratch of the standalone primitives (sg_scan_u32, sg_sort_pairs,
+// sg_bounce): one buffer per stream, so calls on different streams never
+// share device memory; calls on one stream are ordered by the stream.  A
+// buffer that a larger call outgrows is retired, not freed, until the
+// context is destroyed: work already enqueued, or captured into a graph,
+// may still address it.
+struct ScratchBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
 struct sg_ctx {
     int device;
-    // scratch for the standalone primitives (sg_scan_u32, sg_sort_pairs, sg_bounce)
-    void* scratch = nullptr;
-    size_t scratch_bytes = 0;
+    std::mutex scratch_mu;
+    std::map<hipStream_t, ScratchBuf> scratch;   // per stream (nullptr = the null stream)
+    std::vector<void*> retired;
 };
 
 struct sg_scene {
@@ -68,6 +83,21 @@ struct sg_grid {
 // thread; an entry point that checks its own launches with it first drops
 // what earlier, unrelated calls left behind
 inline void clear_last_error() { (void)hipGetLastError(); }
+
+// the status a caller sees for a grid's device status bits
+inline sg_status status_from_bits(uint32_t b) {
+    if (b & ST_INVALID) return SG_ERR_INVALID_SCENE;
+    if (b & ST_TOO_MANY_CELLS) return SG_ERR_TOO_MANY_CELLS;
+    if (b & (ST_CAP_P1 | ST_CAP_C1 | ST_CAP_C2 | ST_CAP_P2)) return SG_ERR_CAPACITY;
+    if (b & ST_NEG_EXTENT) return SG_ERR_NEG_EXTENT;
+    if (b & ST_LEAF_RES) return SG_ERR_LEAF_RES;
+    return SG_OK;
+}
+
+// status bits with which the traversal treats every ray as a miss (a build
+// that failed or outgrew its buffers); grids with a leaf resolution outside
+// [1,255] or a negative leaf extent are traversed as the reference would
+inline bool untraversable(uint32_t b) { return (b & ~(ST_LEAF_RES | ST_NEG_EXTENT)) != 0; }
 
 inline sg_status hip_status(hipError_t e) {
     if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return SG_ERR_OUT_OF_MEMORY;

@@ -152,6 +152,7 @@ struct TraceItem {
     RaySoA rays;
     uint32_t* prim;
     float* t;
+    uint32_t* status;   // optional: receives the grid's status bits (GridHeader::status)
     uint32_t n;
     uint32_t chunk0;
 };

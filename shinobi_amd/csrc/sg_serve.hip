@@ -39,6 +39,7 @@ struct sg_batch {
     uint32_t next_chunk = 0;
     TraceItem* pinned = nullptr;   // staging of the table for the upload
     TraceItem* dev = nullptr;      // device table
+    uint32_t* dev_status = nullptr;// per item: the grid's status bits of the last launch
     hipEvent_t uploaded = nullptr; // the last upload out of `pinned`
     ItemSplit dev_split;           // layout of the device table
 };
@@ -54,9 +55,11 @@ sg_status sg_batch_create(sg_ctx* ctx, uint32_t max_items, sg_batch** out) {
     b->items.reserve(max_items);
     if (hipHostMalloc((void**)&b->pinned, sizeof(TraceItem) * max_items, hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void**)&b->dev, sizeof(TraceItem) * max_items) != hipSuccess ||
+        hipMalloc((void**)&b->dev_status, sizeof(uint32_t) * max_items) != hipSuccess ||
         hipEventCreate(&b->uploaded) != hipSuccess) {
         if (b->pinned) hipHostFree(b->pinned);
         if (b->dev) hipFree(b->dev);
+        if (b->dev_status) hipFree(b->dev_status);
         delete b;
         return SG_ERR_OUT_OF_MEMORY;
     }
@@ -71,6 +74,7 @@ sg_status sg_batch_destroy(sg_batch* b) {
     hipEventDestroy(b->uploaded);
     hipHostFree(b->pinned);
     hipFree(b->dev);
+    hipFree(b->dev_status);
     delete b;
     return SG_OK;
 }
@@ -93,6 +97,7 @@ sg_status sg_batch_add(sg_batch* b, sg_grid* g, const sg_rays* rays, uint32_t n,
     it.rays = ray_soa(rays);
     it.prim = d_prim;
     it.t = d_t;
+    it.status = b->dev_status + b->items.size();
     it.n = n;
     it.chunk0 = b->next_chunk;
     b->next_chunk += chunks_of(n);
@@ -107,7 +112,22 @@ sg_status sg_batch_commit(sg_batch* b, void* stream) {
     const size_t bytes = sizeof(TraceItem) * b->items.size();
     b->dev_split = split_items(b->items.data(), (uint32_t)b->items.size(), b->pinned);
     if (bytes) HIPCHK(hipMemcpyAsync(b->dev, b->pinned, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    // items without rays are never traced: their status reads 0
+    if (!b->items.empty())
+        HIPCHK(hipMemsetAsync(b->dev_status, 0, sizeof(uint32_t) * b->items.size(), (hipStream_t)stream));
     HIPCHK(hipEventRecord(b->uploaded, (hipStream_t)stream));
+    return SG_OK;
+}
+
+sg_status sg_batch_get_status(sg_batch* b, void* stream, sg_status* out, uint32_t n) {
+    if (!b || (!out && n) || n > b->items.size()) return SG_ERR_INVALID_ARG;
+    if (n == 0) return SG_OK;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    std::vector<uint32_t> bits(n);
+    HIPCHK(hipMemcpyAsync(bits.data(), b->dev_status, sizeof(uint32_t) * n, hipMemcpyDeviceToHost,
+                          (hipStream_t)stream));
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    for (uint32_t i = 0; i < n; ++i) out[i] = untraversable(bits[i]) ? status_from_bits(bits[i]) : SG_OK;
     return SG_OK;
 }
 
@@ -232,7 +252,7 @@ sg_status sg_autotune(sg_ctx* ctx, sg_scene* scene, const sg_rays* sample, uint3
             best_cost = cost;
         }
     };
-    // 1. density (radix 8, 256 threads)
+    // 1. density (default radix width 8 and block size 64)
     if (kind_mask & 1) {
         for (float lam : {1.f, 2.f, 3.f, 5.f, 8.f}) {
             sg_build_params p;
@@ -260,7 +280,8 @@ sg_status sg_autotune(sg_ctx* ctx, sg_scene* scene, const sg_rays* sample, uint3
             consider(p);
         }
         const sg_build_params base2 = best;
-        for (int blk : {64, 128}) {
+        for (int blk : {64, 128, 256}) {
+            if (blk == base2.trace_block) continue;
             sg_build_params p = base2;
             p.trace_block = blk;
             consider(p);
@@ -293,6 +314,8 @@ struct Request {
     float* t_out = nullptr;
     size_t bytes = 0;            // device bytes while in flight
     int gi = 0;                  // grid of a dynamic scene the request runs on
+    GridView view{};             // that grid, read under the server lock at admission
+    int trace_block = 64;        // its traversal block size (plain mode)
     Clock::time_point t_submit, t_done;
     sg_status status = SG_OK;
     bool done = false;
@@ -303,6 +326,8 @@ struct Slot {
     hipEvent_t ev = nullptr;
     TraceItem* pinned_items = nullptr;
     TraceItem* dev_items = nullptr;
+    uint32_t* dev_status = nullptr;      // per request: its grid's status bits (batched mode)
+    uint32_t* pinned_status = nullptr;   // the same, copied back with the results
     bool busy = false;
     std::vector<Request*> reqs;
     size_t off = 0, bytes = 0, rays_bytes = 0;   // arena region of the wave
@@ -314,6 +339,8 @@ struct SceneEntry {
     sg_scene* mesh[2] = {nullptr, nullptr};
     int front = 0;
     bool swap_pending = false;      // back grid rebuilt on the rebuild stream
+    bool updating = false;          // an update_dynamic call owns the back copy
+    sg_build_params params{};       // build parameters of a dynamic scene
     hipEvent_t rebuilt = nullptr;
     uint32_t in_use[2] = {0, 0};    // admitted requests that read each grid
 };
@@ -363,7 +390,9 @@ struct sg_server {
     char* dev = nullptr;       // device arena (hbm_budget bytes)
     Arena arena;
     std::vector<Slot> slots;
-    std::vector<SceneEntry> scenes;
+    // a deque: adding a scene never moves the entries that waves and
+    // update_dynamic calls refer to
+    std::deque<SceneEntry> scenes;
     hipStream_t rebuild_st = nullptr;
 
     std::mutex mu;
@@ -378,6 +407,7 @@ struct sg_server {
     // statistics since the last reset
     std::vector<double> lat_us;
     uint64_t st_rays = 0, st_waves = 0;
+    uint64_t regrown = 0;   // dynamic rebuilds that outgrew their grid and were built again
     bool have_first = false;
     Clock::time_point first_submit, last_done;
 };
@@ -388,7 +418,7 @@ namespace {
 // a request's seven arrays are one contiguous SoA block), trace, and copy
 // the results straight back; the copies are DMA transfers when the callers'
 // memory is pinned (sg_host_alloc)
-sg_status launch_wave(sg_server* s, Slot& sl) {
+sg_status launch_wave_body(sg_server* s, Slot& sl) {
     clear_last_error();
     const bool plain = s->cfg.mode == SG_SERVE_PLAIN;
     char* dbase = s->dev + sl.off;
@@ -418,19 +448,19 @@ sg_status launch_wave(sg_server* s, Slot& sl) {
             for (int c = 0; c < 7; ++c)
                 HIPCHK(hipMemcpyAsync(dbase + rofs[k] + c * a, r->rays[c], nb, hipMemcpyHostToDevice, sl.st));
         }
-        const SceneEntry& se = s->scenes[r->scene];
-        const int gi = r->gi;
         TraceItem it;
-        it.g = grid_view(se.grid[gi]);
+        it.g = r->view;
         for (int c = 0; c < 7; ++c) it.rays.a[c] = reinterpret_cast<const float*>(dbase + rofs[k] + c * a);
         it.prim = reinterpret_cast<uint32_t*>(dbase + oofs[k]);
         it.t = reinterpret_cast<float*>(dbase + oofs[k] + a);
+        it.status = sl.dev_status + k;
         it.n = r->n;
         it.chunk0 = chunk;
         chunk += chunks_of(r->n);
         if (plain) {
-            launch_trace(it.g, it.rays, it.n, nullptr, it.prim, it.t, nullptr, false, se.grid[gi]->p.trace_block,
-                         sl.st);
+            launch_trace(it.g, it.rays, it.n, nullptr, it.prim, it.t, nullptr, false, r->trace_block, sl.st);
+            HIPCHK(hipMemcpyAsync(sl.pinned_status + k, &it.g.hdr->status, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                  sl.st));
         } else {
             wave_items[k] = it;
         }
@@ -440,6 +470,8 @@ sg_status launch_wave(sg_server* s, Slot& sl) {
         HIPCHK(hipMemcpyAsync(sl.dev_items, sl.pinned_items, sizeof(TraceItem) * sl.reqs.size(),
                               hipMemcpyHostToDevice, sl.st));
         launch_trace_items(sl.dev_items, split, false, sl.st);
+        HIPCHK(hipMemcpyAsync(sl.pinned_status, sl.dev_status, sizeof(uint32_t) * sl.reqs.size(),
+                              hipMemcpyDeviceToHost, sl.st));
     }
     HIPCHK(hipGetLastError());
     for (size_t k = 0; k < sl.reqs.size(); ++k) {
@@ -452,6 +484,21 @@ sg_status launch_wave(sg_server* s, Slot& sl) {
     return SG_OK;
 }
 
+// A wave that fails part way may have enqueued copies and kernels that
+// still read and write its arena region and the callers' buffers: wait for
+// them and record the slot's event on every exit path, so the slot is
+// released only once the stream has drained.
+sg_status launch_wave(sg_server* s, Slot& sl) {
+    const sg_status e = launch_wave_body(s, sl);
+    if (e != SG_OK) {
+        (void)hipStreamSynchronize(sl.st);
+        (void)hipEventRecord(sl.ev, sl.st);
+        (void)hipEventSynchronize(sl.ev);
+        clear_last_error();
+    }
+    return e;
+}
+
 void serve_loop(sg_server* s) {
     hipSetDevice(s->ctx->device);
     std::unique_lock<std::mutex> lk(s->mu);
@@ -462,7 +509,12 @@ void serve_loop(sg_server* s) {
             if (!sl.busy) continue;
             if (hipEventQuery(sl.ev) != hipSuccess) continue;
             const auto now = Clock::now();
-            for (Request* r : sl.reqs) {
+            for (size_t k = 0; k < sl.reqs.size(); ++k) {
+                Request* r = sl.reqs[k];
+                // a grid that failed or outgrew its buffers was traversed as
+                // all misses: the request fails with the grid's status
+                if (r->status == SG_OK && untraversable(sl.pinned_status[k]))
+                    r->status = status_from_bits(sl.pinned_status[k]);
                 s->scenes[r->scene].in_use[r->gi]--;
                 r->t_done = now;
                 r->done = true;
@@ -520,6 +572,8 @@ void serve_loop(sg_server* s) {
                     SceneEntry& se = s->scenes[r->scene];
                     r->gi = se.dynamic ? se.front : 0;
                     se.in_use[r->gi]++;
+                    r->view = grid_view(se.grid[r->gi]);
+                    r->trace_block = se.grid[r->gi]->p.trace_block;
                 }
                 sl.off = off;
                 sl.bytes = bytes;
@@ -572,7 +626,10 @@ sg_status sg_server_create(sg_ctx* ctx, const sg_server_config* cfg, sg_server**
              hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) == hipSuccess &&
              hipHostMalloc((void**)&sl.pinned_items, sizeof(TraceItem) * s->cfg.max_requests,
                            hipHostMallocDefault) == hipSuccess &&
-             hipMalloc((void**)&sl.dev_items, sizeof(TraceItem) * s->cfg.max_requests) == hipSuccess;
+             hipMalloc((void**)&sl.dev_items, sizeof(TraceItem) * s->cfg.max_requests) == hipSuccess &&
+             hipHostMalloc((void**)&sl.pinned_status, sizeof(uint32_t) * s->cfg.max_requests,
+                           hipHostMallocDefault) == hipSuccess &&
+             hipMalloc((void**)&sl.dev_status, sizeof(uint32_t) * s->cfg.max_requests) == hipSuccess;
     }
     if (!ok) {
         s->stop = true;
@@ -600,6 +657,8 @@ sg_status sg_server_destroy(sg_server* s) {
         if (sl.ev) hipEventDestroy(sl.ev);
         if (sl.pinned_items) hipHostFree(sl.pinned_items);
         if (sl.dev_items) hipFree(sl.dev_items);
+        if (sl.pinned_status) hipHostFree(sl.pinned_status);
+        if (sl.dev_status) hipFree(sl.dev_status);
     }
     for (SceneEntry& se : s->scenes) {
         if (!se.dynamic) continue;
@@ -633,6 +692,7 @@ sg_status sg_server_add_dynamic(sg_server* s, const float* const soa9[9], uint32
     HIPCHK(hipSetDevice(s->ctx->device));
     SceneEntry se;
     se.dynamic = true;
+    se.params = *p;
     sg_status r = SG_OK;
     for (int b = 0; b < 2 && r == SG_OK; ++b) {
         r = sg_scene_create(s->ctx, soa9, n, 0, s->rebuild_st, &se.mesh[b]);
@@ -657,6 +717,7 @@ sg_status sg_server_update_dynamic(sg_server* s, uint32_t id, const float* const
     if (!s || !soa9) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(s->ctx->device));
     int back;
+    SceneEntry* sep = nullptr;
     {
         std::unique_lock<std::mutex> lk(s->mu);
         if (id >= s->scenes.size() || !s->scenes[id].dynamic) return SG_ERR_INVALID_ARG;
@@ -665,16 +726,43 @@ sg_status sg_server_update_dynamic(sg_server* s, uint32_t id, const float* const
         // overwritten; new requests only take the front grid
         s->cv_done.wait(lk, [&] {
             const SceneEntry& se = s->scenes[id];
-            return !se.swap_pending && se.in_use[1 - se.front] == 0;
+            return !se.updating && !se.swap_pending && se.in_use[1 - se.front] == 0;
         });
+        s->scenes[id].updating = true;
         back = 1 - s->scenes[id].front;
+        sep = &s->scenes[id];
     }
-    SceneEntry& se = s->scenes[id];
+    // the entry does not move (deque) and, while `updating` is set, only
+    // this call touches its back mesh and grid (waves read the front one)
+    SceneEntry& se = *sep;
     sg_status r = sg_scene_update(se.mesh[back], soa9, 0, s->rebuild_st);
     if (r == SG_OK) r = sg_rebuild(s->ctx, se.mesh[back], se.grid[back], s->rebuild_st);
-    if (r != SG_OK) return r;
-    HIPCHK(hipEventRecord(se.rebuilt, s->rebuild_st));
+    // A mesh that deformed past the grid's capacity slack overflows the
+    // rebuild (the grid would be traversed as all misses): build the back
+    // grid again, sized for the new mesh, before it is swapped in.
+    sg_grid_info info;
+    if (r == SG_OK) r = sg_grid_get_info(se.grid[back], &info);
+    if (r == SG_OK && info.status == SG_ERR_CAPACITY) {
+        sg_grid* g2 = nullptr;
+        r = sg_build(s->ctx, se.mesh[back], &se.params, s->rebuild_st, &g2);
+        if (r == SG_OK) {
+            sg_grid_destroy(se.grid[back]);
+            std::lock_guard<std::mutex> g(s->mu);
+            se.grid[back] = g2;
+            ++s->regrown;
+        }
+    } else if (r == SG_OK && info.status != SG_OK && info.status != SG_ERR_LEAF_RES &&
+               info.status != SG_ERR_NEG_EXTENT) {
+        r = info.status;   // the new mesh cannot be gridded: keep the front one
+    }
+    if (r == SG_OK && hipEventRecord(se.rebuilt, s->rebuild_st) != hipSuccess) r = SG_ERR_HIP;
     std::unique_lock<std::mutex> lk(s->mu);
+    if (r != SG_OK) {
+        se.updating = false;
+        s->cv_done.notify_all();
+        return r;
+    }
+    se.updating = false;
     se.swap_pending = true;
     s->cv_work.notify_all();
     // returns once requests admitted from now on see the new mesh
@@ -770,6 +858,7 @@ sg_status sg_server_get_stats(sg_server* s, sg_server_stats* out) {
         out->wall_s = std::chrono::duration<double>(s->last_done - s->first_submit).count();
         out->mrays_per_s = out->wall_s > 0 ? (double)s->st_rays / out->wall_s / 1e6 : 0.0;
     }
+    out->regrown = s->regrown;
     return SG_OK;
 }
 

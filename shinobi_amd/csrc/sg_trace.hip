@@ -708,6 +708,9 @@ __global__ __launch_bounds__(kBlock) SG_TRACE_OCCUPANCY void k_trace_items(const
 #pragma unroll
         for (int k = 0; k < (int)(sizeof(TraceItem) / 4); ++k) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
     }
+    // the item's first wave reports the status of the grid it traced
+    // against (a failed or overflowed build is traversed as all misses)
+    if (it.status && c == it.chunk0 && lane_id() == 0) *it.status = it.g.hdr->status;
     const uint32_t i = (c - it.chunk0) * 64u + (uint32_t)lane_id();
     if (i < it.n) {
         // the table's pointers all address device memory
