@@ -39,7 +39,9 @@ enum {
     SG_ERR_TIMEOUT = 9,         /* a device-side wait exceeded its bound           */
     SG_ERR_BUDGET = 10,         /* request exceeds the serving HBM budget          */
     SG_ERR_NO_DEVICE = 11,      /* no HIP device                                   */
-    SG_ERR_IO = 12              /* a scene or camera file cannot be read           */
+    SG_ERR_IO = 12,             /* a scene or camera file cannot be read           */
+    SG_ERR_SCRATCH_LIMIT = 13   /* input larger than an allocation-free entry
+                                   point's fixed scratch admits                   */
 };
 
 const char *sg_status_str(sg_status s);
@@ -192,6 +194,20 @@ sg_status sg_trace_shadow(sg_ctx *ctx, sg_grid *grid, const sg_rays *rays, uint3
 sg_status sg_bounce(sg_ctx *ctx, sg_scene *scene, const sg_rays *rays, const uint32_t *d_prim,
                     const float *d_t, uint32_t n, const uint32_t *d_n, uint64_t seed, const sg_rays_out *out,
                     uint32_t *d_src, uint32_t *d_count, void *stream);
+/* sg_bounce that allocates nothing: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as sg_bounce; its scratch is a fixed device array of the library
+ * (one per GPU, about 16 MiB, held until the process ends) instead of a
+ * per-stream buffer, so n (the input capacity) is limited to
+ * SG_BOUNCE_NOALLOC_MAX_RAYS; a larger n returns SG_ERR_SCRATCH_LIMIT and
+ * enqueues nothing.  Every call on one GPU uses that GPU's one array: calls
+ * whose device work overlaps (different streams, or graphs replayed
+ * concurrently) race and must be ordered by the caller. */
+#define SG_BOUNCE_NOALLOC_MAX_RAYS 16777216u
+sg_status sg_bounce_noalloc(sg_ctx *ctx, sg_scene *scene, const sg_rays *rays, const uint32_t *d_prim,
+                            const float *d_t, uint32_t n, const uint32_t *d_n, uint64_t seed,
+                            const sg_rays_out *out, uint32_t *d_src, uint32_t *d_count, void *stream);
 
 /* ---------------------------------------------------------- primitives --- */
 /* Scratch of sg_bounce, sg_scan_u32 and sg_sort_pairs: the context keeps one

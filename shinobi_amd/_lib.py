@@ -17,7 +17,10 @@ STATUS_NAMES = {
     0: "SG_OK", 1: "SG_ERR_INVALID_ARG", 2: "SG_ERR_INVALID_SCENE", 3: "SG_ERR_OUT_OF_MEMORY",
     4: "SG_ERR_HIP", 5: "SG_ERR_CAPACITY", 6: "SG_ERR_LEAF_RES", 7: "SG_ERR_TOO_MANY_CELLS",
     8: "SG_ERR_NEG_EXTENT", 9: "SG_ERR_TIMEOUT", 10: "SG_ERR_BUDGET", 11: "SG_ERR_NO_DEVICE",
+    12: "SG_ERR_IO", 13: "SG_ERR_SCRATCH_LIMIT",
 }
+SG_ERR_SCRATCH_LIMIT = 13
+SG_BOUNCE_NOALLOC_MAX_RAYS = 16777216
 
 SG_GRID_UNIFORM = 0
 SG_GRID_TWO_LEVEL = 1
@@ -125,6 +128,8 @@ SIGNATURES = {
                                   C.c_void_p, C.c_void_p, C.c_void_p]),
     "sg_bounce": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_uint32,
                             C.c_void_p, C.c_uint64, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sg_bounce_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_uint32,
+                                    C.c_void_p, C.c_uint64, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_void_p]),
     "sg_scan_u32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]),
     "sg_sort_pairs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p]),
     "sg_malloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),

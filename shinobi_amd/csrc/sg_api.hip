@@ -18,6 +18,39 @@ static int bit_length(uint64_t x) {
     return b < 1 ? 1 : b;
 }
 
+// Scratch of sg_bounce_noalloc: a fixed device array, sized for
+// SG_BOUNCE_NOALLOC_MAX_RAYS input rays (the per-tile class counts and class
+// totals of the bounce ordering; every word is written before it is read, so
+// nothing needs clearing between calls).  Each GPU holds its own copy at its
+// own address until the process ends.  In host code `scratch` names a
+// host-side stand-in, not device memory: its device address on each GPU is
+// looked up once by sg_ctx_create (noalloc_init), and only that address is
+// ever passed to kernels.
+static constexpr size_t kNoAllocScratchBytes = 16777728;   // >= bounce_scratch_bytes(MAX), multiple of 256
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[kMaxDevices];   // device address of `scratch`, per device
+
+// with `device` current: find the array's address on it (once per device)
+static sg_status noalloc_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return SG_OK;   // sg_bounce_noalloc then reports SG_ERR_HIP
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return SG_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return e != hipSuccess ? hip_status(e) : SG_ERR_HIP;
+    g_noalloc_ptr[device] = p;
+    return SG_OK;
+}
+
+static void* noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    return g_noalloc_ptr[device];
+}
+
 extern "C" {
 
 const char* sg_status_str(sg_status s) {
@@ -35,6 +68,7 @@ const char* sg_status_str(sg_status s) {
         case SG_ERR_BUDGET: return "request exceeds the HBM budget";
         case SG_ERR_NO_DEVICE: return "no HIP device";
         case SG_ERR_IO: return "file cannot be read";
+        case SG_ERR_SCRATCH_LIMIT: return "input exceeds the fixed scratch of an allocation-free entry point";
         default: return "unknown status";
     }
 }
@@ -65,6 +99,8 @@ sg_status sg_ctx_create(int device, sg_ctx** out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return SG_ERR_NO_DEVICE;
     if (device < 0 || device >= n) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(device));
+    const sg_status ns = noalloc_init(device);
+    if (ns != SG_OK) return ns;
     sg_ctx* c = new sg_ctx();
     c->device = device;
     *out = c;
@@ -583,8 +619,8 @@ sg_status sg_bounce(sg_ctx* ctx, sg_scene* scene, const sg_rays* rays, const uin
     if (!ctx || !scene || !rays || !d_prim || !d_t || !out || !d_count) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
     clear_last_error();
-    void* scratch = nullptr;
-    sg_status r = ctx_scratch(ctx, (hipStream_t)stream, bounce_scratch_bytes(n), &scratch);
+    void* buf = nullptr;
+    sg_status r = ctx_scratch(ctx, (hipStream_t)stream, bounce_scratch_bytes(n), &buf);
     if (r != SG_OK) return r;
     RaySoAOut o;
     for (int i = 0; i < 3; ++i) {
@@ -592,7 +628,32 @@ sg_status sg_bounce(sg_ctx* ctx, sg_scene* scene, const sg_rays* rays, const uin
         o.a[3 + i] = out->d[i];
     }
     o.a[6] = out->tmax;
-    launch_bounce(scene_soa(scene), ray_soa(rays), d_prim, d_t, n, d_n, seed, o, d_src, d_count, scratch,
+    launch_bounce(scene_soa(scene), ray_soa(rays), d_prim, d_t, n, d_n, seed, o, d_src, d_count, buf,
+                  (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return SG_OK;
+}
+
+// sg_bounce with the library's fixed per-device scratch array: no allocation
+// of any kind, so it may be called under graph capture at any size up to
+// the limit
+sg_status sg_bounce_noalloc(sg_ctx* ctx, sg_scene* scene, const sg_rays* rays, const uint32_t* d_prim,
+                            const float* d_t, uint32_t n, const uint32_t* d_n, uint64_t seed, const sg_rays_out* out,
+                            uint32_t* d_src, uint32_t* d_count, void* stream) {
+    if (!ctx || !scene || !rays || !d_prim || !d_t || !out || !d_count) return SG_ERR_INVALID_ARG;
+    if (n > SG_BOUNCE_NOALLOC_MAX_RAYS || bounce_scratch_bytes(n) > kNoAllocScratchBytes)
+        return SG_ERR_SCRATCH_LIMIT;
+    HIPCHK(hipSetDevice(ctx->device));
+    void* dev_scratch = noalloc_scratch(ctx->device);
+    if (!dev_scratch) return SG_ERR_HIP;
+    clear_last_error();
+    RaySoAOut o;
+    for (int i = 0; i < 3; ++i) {
+        o.a[i] = out->o[i];
+        o.a[3 + i] = out->d[i];
+    }
+    o.a[6] = out->tmax;
+    launch_bounce(scene_soa(scene), ray_soa(rays), d_prim, d_t, n, d_n, seed, o, d_src, d_count, dev_scratch,
                   (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return SG_OK;
@@ -603,10 +664,10 @@ sg_status sg_scan_u32(sg_ctx* ctx, const uint32_t* d_in, uint32_t* d_out, uint32
     if (!ctx || (!d_in && n) || (!d_out && n)) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(ctx->device));
     clear_last_error();
-    void* scratch = nullptr;
-    sg_status r = ctx_scratch(ctx, (hipStream_t)stream, scan_scratch_bytes(n), &scratch);
+    void* buf = nullptr;
+    sg_status r = ctx_scratch(ctx, (hipStream_t)stream, scan_scratch_bytes(n), &buf);
     if (r != SG_OK) return r;
-    launch_scan(d_in, d_out, n, inclusive != 0, scratch, (hipStream_t)stream);
+    launch_scan(d_in, d_out, n, inclusive != 0, buf, (hipStream_t)stream);
     HIPCHK(hipGetLastError());
     return SG_OK;
 }
@@ -628,11 +689,11 @@ sg_status sg_sort_pairs(sg_ctx* ctx, uint32_t* d_keys, uint32_t* d_vals, uint32_
     // scratch: [64 words: n][keys n][vals n][sort scratch]
     const size_t sw = sort_scratch_words(n, key_bits, radix_bits);
     const size_t bytes = 256 + align_up((size_t)n * 4, 256) * 2 + sw * 4;
-    void* scratch = nullptr;
-    sg_status r = ctx_scratch(ctx, st, bytes, &scratch);
+    void* buf = nullptr;
+    sg_status r = ctx_scratch(ctx, st, bytes, &buf);
     if (r != SG_OK) return r;
-    uint32_t* words = (uint32_t*)scratch;
-    uint32_t* tk = (uint32_t*)((char*)scratch + 256);
+    uint32_t* words = (uint32_t*)buf;
+    uint32_t* tk = (uint32_t*)((char*)buf + 256);
     uint32_t* tv = (uint32_t*)((char*)tk + align_up((size_t)n * 4, 256));
     uint32_t* ss = (uint32_t*)((char*)tv + align_up((size_t)n * 4, 256));
     hipLaunchKernelGGL(k_sort_setup, dim3(1), dim3(64), 0, st, words, n);
