@@ -8,8 +8,10 @@ Workload (one step = one pass of the hot path over one batch):
             resolution-fixed top level lambda 5 / 6, leaf lambda 2, every
             phase on the device, no host sync)
          2. closest-hit trace of 1920x1080 primary rays from (0.5,-0.3,0.6),
-            ordered in 2x32 pixel tiles (the row-major time is reported as
-            trace_primary_rowmajor_ms)
+            generated inside the trace kernel (sg_trace_camera: the
+            reference's Camera + RegularPixelSampler per pixel) and visited
+            in 2x32 pixel tiles; hits and rays are written at their pixel
+            ids (the row-major visit is reported as trace_primary_rowmajor_ms)
          3. one cosine-weighted diffuse bounce per hit (device generator,
             compacted and ordered by direction class in rounds of 256)
          4. closest-hit trace of the bounce rays (count read on the device)
@@ -36,9 +38,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 FP32_PEAK_TFLOPS = 157.3       # MI355X vector FP32, TFLOP/s
 
-# Primary rays are traced in 2x32-pixel column tiles (workloads.tile_order):
-# a wave then holds near and far rays, and the lanes that finish early test
-# the remaining rays' cells with them (wave-shared tests).  Measured on the
+# The camera trace visits the pixels in 2x32-pixel column tiles: a wave then
+# holds near and far rays, and the lanes that finish early test the
+# remaining rays' cells with them (wave-shared tests).  Measured on the
 # height field: 2x32 0.51 ms, 4x16 0.56, 8x8 0.59, row-major 0.62; on the
 # ruins camera 2x32 0.55, 8x8 0.60, row-major 0.70.
 PRIMARY_TILE = (2, 32)
@@ -85,25 +87,70 @@ def traffic_of(kernel: str):
     return None if k is None else int(k["fetch_bytes"] + k["write_bytes"])
 
 
-def cpu_baseline(w, n_primary: int, seed: int, threads: int) -> dict:
+def host_cpu() -> dict:
+    """the host's CPU model and the cores this process may use: its affinity
+    mask, limited by a cgroup CPU quota when one is set"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cores = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        pass
+    return {"model": model, "affinity_cores": cores, "cgroup_quota_cores": quota,
+            "cores": min(cores, quota) if quota else cores}
+
+
+def cpu_baseline(w, n_primary: int, seed: int, threads: int, runs: int = 5) -> dict:
+    """The oracle (the restatement of the reference's GPU path, compiled
+    -O2 for the host) on this host's cores: two-level build, closest-hit
+    trace of the primary rays, one bounce per hit, trace of the bounce rays.
+    Median of `runs` runs after one warm-up, with all usable cores and with
+    one core."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import py_oracle as O
     O.load()
-    t0 = time.perf_counter()
-    og = O.build_twolevel(w.scene, nthreads=threads, **w.oracle)
-    t_build = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    prim, t = og.trace(w.rays, nthreads=threads)
-    t_p = time.perf_counter() - t0
-    b, _ = O.bounce(w.scene, w.rays, prim, t, seed)
-    t0 = time.perf_counter()
-    og.trace(b, nthreads=threads)
-    t_b = time.perf_counter() - t0
-    rays = n_primary + b.shape[1]
-    return {"value": round(rays / (t_p + t_b) / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "build_ms": round(t_build * 1e3, 1),
+
+    def once(nt):
+        t0 = time.perf_counter()
+        og = O.build_twolevel(w.scene, nthreads=nt, **w.oracle)
+        t_build = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        prim, t = og.trace(w.rays, nthreads=nt)
+        t_p = time.perf_counter() - t0
+        b, _ = O.bounce(w.scene, w.rays, prim, t, seed)
+        t0 = time.perf_counter()
+        og.trace(b, nthreads=nt)
+        t_b = time.perf_counter() - t0
+        og.close()
+        return t_build, t_p + t_b, b.shape[1]
+
+    res = {}
+    for nt in (threads, 1):
+        once(nt)
+        rows = [once(nt) for _ in range(runs)]
+        t_build = float(np.median([r[0] for r in rows]))
+        t_trace = float(np.median([r[1] for r in rows]))
+        rays = n_primary + rows[0][2]
+        res[nt] = (round(rays / t_trace / 1e6, 3), round(t_build * 1e3, 1), rows[0][2])
+    cpu = host_cpu()
+    return {"value": res[threads][0], "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "build_ms": res[threads][1], "one_core": {"value": res[1][0], "unit": "Mrays/s", "build_ms": res[1][1]},
+            "cpu_model": cpu["model"], "host": cpu, "runs": f"median of {runs} after 1 warm-up",
             "sample": f"full workload on the host: oracle two-level build + closest-hit trace of {n_primary} "
-                      f"primary and {b.shape[1]} bounce rays, {threads} threads"}
+                      f"primary and {res[threads][2]} bounce rays (row-major pixel order), {threads} threads "
+                      f"and 1 thread"}
 
 
 def _log_uniform(rng, lo, hi, n):
@@ -234,16 +281,18 @@ def cluster_bench(sa, W, ctx, spp: int = 4, depth: int = 4, reps: int = 3) -> di
 def ruins_bench(sa, W, ctx, reps: int = 10) -> dict:
     """The reference's own scene (scenes/ruins: 38,665 triangles, its
     1024x1024 camera.cfg view): two-level rebuild with the reference
-    builder's defaults, primary trace, one diffuse bounce, bounce trace."""
+    builder's defaults, camera trace (rays generated in the kernel), one
+    diffuse bounce, bounce trace."""
     golden = os.path.join(ROOT, "tests", "golden")
-    w = W.scene_file(os.path.join(golden, "ruins.obj.gz"), os.path.join(golden, "ruins_camera.cfg"), tile=PRIMARY_TILE)
+    w = W.scene_file(os.path.join(golden, "ruins.obj.gz"), os.path.join(golden, "ruins_camera.cfg"))
     scene = sa.FaceSoup(ctx, w.scene)
     b = sa.FastTwoLevelGridBuilder(ctx, **w.build)
     b.init(scene)
     g = b.build()
     info = g.info()
     n = w.rays.shape[1]
-    rays = sa.RayBatch(ctx, w.rays)
+    cam = sa.pinhole(*w.camera)
+    rays = sa.RayBatch(ctx, capacity=n)
     prim, t = sa.DeviceBuffer(ctx, 4 * n), sa.DeviceBuffer(ctx, 4 * n)
     out = sa.RayBatch(ctx, capacity=n)
     bp, bt = sa.DeviceBuffer(ctx, 4 * n), sa.DeviceBuffer(ctx, 4 * n)
@@ -254,7 +303,7 @@ def ruins_bench(sa, W, ctx, reps: int = 10) -> dict:
         ev[0].record()
         g.rebuild()
         ev[1].record()
-        g.trace(rays, prim=prim, t=t)
+        g.trace_camera(cam, PRIMARY_TILE, prim=prim, t=t, rays_out=rays)
         ev[2].record()
         sa.bounce(ctx, scene, rays, prim, t, w.bounce_seed, out, count=count)
         ev[3].record()
@@ -382,9 +431,10 @@ def main() -> int:
     import shinobi_amd as sa
     from shinobi_amd import workloads as W
 
-    w = W.heightfield2m(tile=PRIMARY_TILE)
+    w = W.heightfield2m()
     n_tris = w.scene.shape[1]
     n_rays = w.rays.shape[1]
+    cam = sa.pinhole(*w.camera)
     ctx = sa.Context(0)
     scene = sa.FaceSoup(ctx, w.scene)
     builder = sa.FastTwoLevelGridBuilder(ctx, **w.build)
@@ -394,7 +444,7 @@ def main() -> int:
     if info.status != 0:
         raise RuntimeError(f"build failed: status {info.status}")
 
-    rays = sa.RayBatch(ctx, w.rays)
+    rays = sa.RayBatch(ctx, capacity=n_rays)      # the camera trace writes the primary rays here
     prim = sa.DeviceBuffer(ctx, 4 * n_rays)
     tbuf = sa.DeviceBuffer(ctx, 4 * n_rays)
     brays = sa.RayBatch(ctx, capacity=n_rays)
@@ -403,9 +453,15 @@ def main() -> int:
     count = sa.DeviceBuffer(ctx, 4)
 
     # untimed pass with traversal counters: the FLOP model of this workload
+    # (per-ray counts, so the order in which the rays are visited does not
+    # change it); the device-generated rays equal the host generator's
     st1 = sa.DeviceBuffer(ctx, 16 * n_rays)
     st2 = sa.DeviceBuffer(ctx, 16 * n_rays)
-    grid.trace(rays, prim=prim, t=tbuf, stats=st1)
+    grid.trace_camera(cam, PRIMARY_TILE, prim=prim, t=tbuf, rays_out=rays)
+    ctx.sync()
+    if not np.array_equal(rays.download().view(np.uint32), w.rays.view(np.uint32)):
+        raise RuntimeError("device camera rays differ from the host generator")
+    grid.trace(rays, stats=st1)
     sa.bounce(ctx, scene, rays, prim, tbuf, w.bounce_seed, brays, count=count)
     grid.trace(brays, prim=bprim, t=bt, stats=st2, d_count=count.ptr, n=n_rays)
     m = int(count.download(np.uint32, 1)[0])
@@ -419,7 +475,7 @@ def main() -> int:
 
     def step(e):
         grid.rebuild_phased(e[:9])
-        grid.trace(rays, prim=prim, t=tbuf)
+        grid.trace_camera(cam, PRIMARY_TILE, prim=prim, t=tbuf, rays_out=rays)
         e[9].record()
         sa.bounce(ctx, scene, rays, prim, tbuf, w.bounce_seed, brays, count=count)
         e[10].record()
@@ -449,20 +505,18 @@ def main() -> int:
         tp_ms.append(e[8].elapsed_ms(e[9]))
         bn_ms.append(e[9].elapsed_ms(e[10]))
         tb_ms.append(e[10].elapsed_ms(e[11]))
-    # the same primary rays in row-major pixel order, for reference
+    # the same primary rays visited in row-major pixel order, for reference
     rowmajor_ms = None
     if not args.no_rowmajor:
-        rays_rm = sa.RayBatch(ctx, W.heightfield_camera(2, 1920, 1080))
         e_a, e_b = ctx.event(), ctx.event()
         rm = []
         for _ in range(6):
             e_a.record()
-            grid.trace(rays_rm, prim=prim, t=tbuf)
+            grid.trace_camera(cam, (1, 1), prim=prim, t=tbuf, rays_out=rays)
             e_b.record()
             ctx.sync()
             rm.append(e_a.elapsed_ms(e_b))
         rowmajor_ms = round(float(np.median(rm[1:])), 4)
-        del rays_rm
     info2 = grid.info()
     if info2.status != 0 or (info2.P1, info2.P2, info2.C2) != (info.P1, info.P2, info.C2):
         raise RuntimeError("rebuild changed the grid")
@@ -480,9 +534,12 @@ def main() -> int:
     phase_gbs = {p: round(bb[p] / (mean(ph[p]) * 1e-3) / 1e9, 1) for p in PHASES}
     build_gbs = total_bytes / (mean(build_ms) * 1e-3) / 1e9
 
-    # roofline of the dominant kernel: k_trace<1,false,false,false>, the two-level
-    # closest-hit traversal, launched twice per step (primary, bounce); the
-    # object describes its longer launch, both launches are listed beside it
+    # roofline of the dominant kernel: the two-level closest-hit traversal,
+    # launched twice per step -- k_trace_gen<1,true> on the camera rays,
+    # k_trace<1,false,false,false> on the bounce rays; the object describes
+    # the longer launch, both launches are listed beside it
+    kernel_of = {"primary": "k_trace_gen<1,true> (two-level closest hit, camera rays generated in the kernel)",
+                 "bounce": "k_trace<1,false,false,false> (two-level closest hit, bounce rays)"}
     launches = {}
     for name, f, ms in (("primary", f_primary, mean(tp_ms)), ("bounce", f_bounce, mean(tb_ms))):
         tf = f / (ms * 1e-3) / 1e12
@@ -491,7 +548,7 @@ def main() -> int:
     dom = max(launches, key=lambda k: launches[k]["ms"])
     dominant = max([("trace_primary", mean(tp_ms)), ("trace_bounce", mean(tb_ms))] +
                    [(p, mean(v)) for p, v in ph.items()], key=lambda x: x[1])
-    roofline = {"kernel": f"k_trace<1,false,false,false> (two-level closest hit, {dom} rays)", "bound": "valu",
+    roofline = {"kernel": kernel_of[dom], "bound": "valu",
                 "achieved": launches[dom]["achieved"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": launches[dom]["frac"], "traffic": launches[dom]["traffic"],
                 "flops_per_launch": launches[dom]["flops"], "launches": launches,
@@ -502,7 +559,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        threads = args.cpu_threads or host_cpu()["cores"]
         cpu = cpu_baseline(w, n_rays, w.bounce_seed, threads)
 
     out = {
@@ -520,6 +577,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "heightfield2m: 2,093,058-tri height field, two-level grid "
                                "(lambda 5 / 6 top, lambda 2 leaf), 1920x1080 primary + 1 diffuse bounce per hit",
+                   "primary_rays_from": "sg_trace_camera (generated in the trace kernel)",
                    "primary_order": f"{PRIMARY_TILE[0]}x{PRIMARY_TILE[1]} pixel tiles",
                    "n_tris": n_tris, "primary_rays": n_rays, "bounce_rays": m, "res_top": list(info.res),
                    "C1": info.C1, "P1": info.P1, "C2": info.C2, "P2": info.P2},

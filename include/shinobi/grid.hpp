@@ -16,6 +16,7 @@
 // binding); FaceSoup::indices is read through grid.indices() after a build.
 #pragma once
 
+#include <math.h>
 #include <stdint.h>
 
 #include <cstring>
@@ -24,6 +25,22 @@
 #include <vector>
 
 #include "../shinobi_grid.h"
+
+// The value types below (vec3f, BBox, Triangle, GridParameters,
+// TwoLevelGridCell) are also the device-side types of shinobi/device.hpp:
+// compiled by hipcc their members are host and device functions.
+#if defined(__HIPCC__)
+#define SHINOBI_HD __host__ __device__ inline
+#else
+#define SHINOBI_HD inline
+#endif
+// First statement of a function whose binary32 arithmetic must round once
+// per operation (no fused multiply-add), whatever -ffp-contract says.
+#if defined(__clang__)
+#define SHINOBI_EXACT _Pragma("clang fp contract(off)")
+#else
+#define SHINOBI_EXACT
+#endif
 
 namespace shinobi {
 
@@ -38,16 +55,51 @@ inline void check(sg_status s, const char* what) {
     if (s != SG_OK) throw Error(s, what);
 }
 
-// src/Core/Algebra.hpp:407-610 (only what the host interface needs)
+// src/Core/Algebra.hpp:407-610: per-component arithmetic, dot product
+// (x*x' + y*y') + z*z', cross product (operator%), component-wise min / max
+// (fminf / fmaxf, as the reference's device code), all in the reference's
+// evaluation order; division is IEEE division (the reference's fastDivide)
 struct vec3f {
     float x, y, z;
-    float& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
-    float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    SHINOBI_HD float& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    SHINOBI_HD float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    SHINOBI_HD static vec3f rep(float f) { return vec3f{f, f, f}; }
+    SHINOBI_HD vec3f operator+(const vec3f& b) const { SHINOBI_EXACT return vec3f{x + b.x, y + b.y, z + b.z}; }
+    SHINOBI_HD vec3f operator-(const vec3f& b) const { SHINOBI_EXACT return vec3f{x - b.x, y - b.y, z - b.z}; }
+    SHINOBI_HD vec3f operator*(const vec3f& b) const { SHINOBI_EXACT return vec3f{x * b.x, y * b.y, z * b.z}; }
+    SHINOBI_HD vec3f operator/(const vec3f& b) const { SHINOBI_EXACT return vec3f{x / b.x, y / b.y, z / b.z}; }
+    SHINOBI_HD vec3f operator*(float f) const { SHINOBI_EXACT return vec3f{x * f, y * f, z * f}; }
+    SHINOBI_HD vec3f operator/(float f) const { SHINOBI_EXACT return vec3f{x / f, y / f, z / f}; }
+    SHINOBI_HD vec3f operator-() const { return vec3f{-x, -y, -z}; }
+    SHINOBI_HD float dot(const vec3f& b) const { SHINOBI_EXACT return x * b.x + y * b.y + z * b.z; }
+    SHINOBI_HD vec3f cross(const vec3f& b) const {
+        SHINOBI_EXACT
+        return vec3f{y * b.z - z * b.y, z * b.x - x * b.z, x * b.y - y * b.x};
+    }
+    SHINOBI_HD vec3f operator%(const vec3f& b) const { return cross(b); }
+    SHINOBI_HD static vec3f min(const vec3f& a, const vec3f& b) {
+        return vec3f{fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)};
+    }
+    SHINOBI_HD static vec3f max(const vec3f& a, const vec3f& b) {
+        return vec3f{fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)};
+    }
+    SHINOBI_HD static vec3f fastDivide(const vec3f& a, const vec3f& b) { return a / b; }
 };
+SHINOBI_HD vec3f operator*(float f, const vec3f& v) { SHINOBI_EXACT return vec3f{f * v.x, f * v.y, f * v.z}; }
 
 // src/Primitive/BBox.hpp:34-125
 struct BBox {
     vec3f min, max;
+    // BBox::fastClip (BBox.hpp:60-75): slab entry / exit distances for rcp(dir)
+    SHINOBI_HD void fastClip(const vec3f& org, const vec3f& dirRCP, float& entry, float& exit) const {
+        SHINOBI_EXACT
+        const vec3f t1 = (min - org) * dirRCP;
+        vec3f tMax = (max - org) * dirRCP;
+        const vec3f tMin = vec3f::min(t1, tMax);
+        tMax = vec3f::max(t1, tMax);
+        entry = fmaxf(fmaxf(tMin.x, tMin.y), tMin.z);
+        exit = fminf(fminf(tMax.x, tMax.y), tMax.z);
+    }
 };
 
 // src/Primitive/Triangle.hpp:33-36
@@ -61,6 +113,8 @@ struct GridParameters {
     vec3f cellSize;
     vec3f cellSizeRCP;
     BBox bounds;
+    SHINOBI_HD vec3f getCellSize() const { return cellSize; }
+    SHINOBI_HD vec3f getCellSizeRCP() const { return cellSizeRCP; }
 };
 
 // src/Control/Structure/TwoLevelGrid.hpp:34-152: identical 8-byte layout,
@@ -68,9 +122,9 @@ struct GridParameters {
 struct TwoLevelGridCell {
     uint32_t x, y;
     enum : uint32_t { SHIFTX = 0, SHIFTY = 8, SHIFTZ = 16, MASK = 0xFF, LARGENOTEMPTYMASK = 0x40000000u };
-    uint32_t notEmpty() const { return y & LARGENOTEMPTYMASK; }
-    uint32_t operator[](uint32_t id) const { return (y >> (id * 8)) & MASK; }
-    uint32_t getLeafRangeBegin() const { return x; }
+    SHINOBI_HD uint32_t notEmpty() const { return y & LARGENOTEMPTYMASK; }
+    SHINOBI_HD uint32_t operator[](uint32_t id) const { return (y >> (id * 8)) & MASK; }
+    SHINOBI_HD uint32_t getLeafRangeBegin() const { return x; }
 };
 static_assert(sizeof(TwoLevelGridCell) == 8, "TwoLevelGridCell must stay 8 bytes");
 
@@ -190,6 +244,13 @@ public:
     }
     // FaceSoup::indices of the reference: reference index -> triangle id
     std::vector<uint32_t> indices() const { return download(SG_ARRAY_INDICES); }
+    // device pointers of the built grid and its scene, for kernels written
+    // against shinobi/device.hpp (waits for the grid's stream)
+    sg_grid_device_view deviceView() const {
+        sg_grid_device_view v;
+        check(sg_grid_get_device_view(grid_, &v), "sg_grid_get_device_view");
+        return v;
+    }
 
 protected:
     void adopt(sg_grid* g) {

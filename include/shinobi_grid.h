@@ -160,6 +160,32 @@ sg_status sg_rebuild_phased(sg_ctx *ctx, sg_scene *scene, sg_grid *grid, void *s
 /* waits for the grid's stream, reads the device header */
 sg_status sg_grid_get_info(sg_grid *grid, sg_grid_info *out);
 sg_status sg_grid_download(sg_grid *grid, int which, void *host, size_t bytes);
+/* Device pointers of a built grid and its scene, for kernels written against
+ * the reference's device concepts (include/shinobi/device.hpp: the storage
+ * FaceSoup's indices / operator[] / operator(), SimpleIntersector, the
+ * grid traversers).  In the reference they read the grid through bound
+ * textures and __constant__ GridParameters (src/Control/Structure/TwoLevelGrid.h:
+ * 36-37, 263-301, src/Control/Structure/FaceSoup.h:36-39); here they read
+ * these arrays.  Waits for the grid's stream (it reads the device header).
+ * The pointers stay valid until the grid (or the scene) is destroyed; a
+ * rebuild rewrites the arrays in place. */
+typedef struct {
+    int kind;                 /* SG_GRID_UNIFORM or SG_GRID_TWO_LEVEL           */
+    sg_status status;         /* the build's status, as sg_grid_get_info        */
+    int res[3];
+    float bmin[3], bmax[3];
+    float cell_size[3];       /* (max-min)/res                                  */
+    float cell_size_rcp[3];   /* res/(max-min)                                  */
+    const uint32_t *cells;    /* 2*n_cells: uniform [begin,end) ranges; two-level
+                                 packed TwoLevelGridCell                        */
+    const uint32_t *leaves;   /* two-level: 2*n_leaves leaf ranges (else NULL)  */
+    uint32_t n_cells, n_leaves;
+    const uint32_t *indices;  /* n_refs: reference -> triangle id (FaceSoup::indices) */
+    uint32_t n_refs;
+    const float *tri[9];      /* the scene's SoA vertices v0.xyz v1.xyz v2.xyz  */
+    uint32_t n_tris;
+} sg_grid_device_view;
+sg_status sg_grid_get_device_view(sg_grid *grid, sg_grid_device_view *out);
 sg_status sg_grid_destroy(sg_grid *grid);
 
 /* ---------------------------------------------------------- traversal --- */
@@ -182,6 +208,45 @@ sg_status sg_trace_closest(sg_ctx *ctx, sg_grid *grid, const sg_rays *rays, uint
 sg_status sg_trace_shadow(sg_ctx *ctx, sg_grid *grid, const sg_rays *rays, uint32_t n,
                           const uint32_t *d_count, uint32_t *d_prim, float *d_t, uint32_t *d_stats,
                           void *stream);
+/* The trace kernel's output buffer, SimpleRayBuffer::store
+ * (src/Engine/RayBuffers.h:52-77): 8 words per ray, SoA at stride n --
+ * hit point org + (t - EPS) * dir (x, y, z), dir (x, y, z), t, bestHit (the
+ * uint32 bits of the reference index the traverser returns: FaceSoup::indices
+ * maps it to the triangle; 0xFFFFFFFF on a miss, where the reference leaves
+ * it uninitialised).  dir is 1 / (1 / d) per component, the direction the
+ * trace kernel restores from rcp(dir) (src/Engine/RayTracingKernels.h:128-130).
+ * d_raybuf: 8 * n floats; d_prim / d_t (optional) as sg_trace_closest. */
+sg_status sg_trace_closest_store(sg_ctx *ctx, sg_grid *grid, const sg_rays *rays, uint32_t n,
+                                 const uint32_t *d_count, float *d_raybuf, uint32_t *d_prim, float *d_t,
+                                 void *stream);
+
+/* A pinhole camera after Camera::init (src/Primitive/Camera.h:41-60) with
+ * the resolution of its RegularPixelSampler<1,1> (src/Primitive/PixelSampler.h:34-54). */
+typedef struct {
+    float position[3];
+    float top_left[3];    /* mTopLeft: forward - row / 2 - col / 2 */
+    float step_x[3];      /* mStepX = row / resX */
+    float step_y[3];      /* mStepY = col / resY */
+    uint32_t res_x, res_y;
+} sg_pinhole;
+/* Camera::init on the host: eye, forward, up, vertical opening angle in degrees */
+sg_status sg_pinhole_init(const float eye[3], const float fwd[3], const float up[3], float fov_deg, uint32_t res_x,
+                          uint32_t res_y, sg_pinhole *out);
+/* Closest hit of the camera's primary rays, generated inside the trace
+ * kernel as RegularPrimaryRayGenerator does (src/Engine/RayGenerators.h:36-61,
+ * called per ray by trace, src/Engine/RayTracingKernels.h:104-106): ray of
+ * pixel id p = x + y * res_x has org = position, dir = topLeft + (frac(p /
+ * resX) resX + 0.5) stepX + (trunc(p / resX) + 0.5) stepY, t = FLT_MAX.
+ * The launch visits the pixels in tiles of tile_w x tile_h (tiles in
+ * row-major order, row-major inside a tile; 1 x 1 is the reference's
+ * row-major order) so that a wave holds a tile's rays; every output is
+ * written at the pixel id, so results do not depend on the tiles.
+ * Outputs, each optional: d_prim / d_t (res_x * res_y each, as
+ * sg_trace_closest), d_raybuf (SimpleRayBuffer, 8 * res_x * res_y floats,
+ * as sg_trace_closest_store), rays_out (the generated rays, for a bounce). */
+sg_status sg_trace_camera(sg_ctx *ctx, sg_grid *grid, const sg_pinhole *cam, uint32_t tile_w, uint32_t tile_h,
+                          uint32_t *d_prim, float *d_t, float *d_raybuf, const sg_rays_out *rays_out, void *stream);
+
 /* one cosine-weighted diffuse bounce per hit (SimplePathTracer.hpp:115-166),
  * compacted and ordered by direction class (4 bins of each component of the
  * new direction, class = bx | by<<2 | bz<<4) in rounds of 256 rays: sorted by

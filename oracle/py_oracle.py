@@ -90,6 +90,7 @@ def load() -> C.CDLL:
     lib.or_grid_free.argtypes = [C.POINTER(GridC)]
     lib.or_trace.argtypes = [C.POINTER(GridC), P9, P7, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
     lib.or_trace_shadow.argtypes = lib.or_trace.argtypes
+    lib.or_trace_store.argtypes = [C.POINTER(GridC), P9, P7, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
     lib.or_brute_f32.argtypes = [P9, C.c_uint32, P7, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int]
     lib.or_brute_f64.argtypes = [P9, C.c_uint32, P7, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int]
     lib.or_bounce.restype = C.c_uint32
@@ -159,6 +160,17 @@ class OracleGrid:
         fn(self._p, _p9(self._soa9), _p7(rays7), n, prim.ctypes.data, t.ctypes.data,
            stats.ctypes.data if with_stats else None, nthreads)
         return (prim, t, stats) if with_stats else (prim, t)
+
+    def trace_store(self, rays7: np.ndarray, nthreads: int = 8) -> np.ndarray:
+        """SimpleRayBuffer words (8, n) float32 of a closest-hit trace"""
+        rays7 = np.ascontiguousarray(rays7, np.float32)
+        n = rays7.shape[1]
+        prim = np.empty(n, np.uint32)
+        t = np.empty(n, np.float32)
+        buf = np.empty((8, n), np.float32)
+        load().or_trace_store(self._p, _p9(self._soa9), _p7(rays7), n, prim.ctypes.data, t.ctypes.data,
+                              buf.ctypes.data, nthreads)
+        return buf
 
     def close(self):
         if self._p:

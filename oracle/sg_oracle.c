@@ -758,7 +758,29 @@ typedef struct {
     const float *const *rays;
     uint32_t *prim; float *t; uint32_t *stats;
     int shadow;             /* any-hit: stop once t < OR_SHADOW_T */
+    float *raybuf;          /* optional SimpleRayBuffer, 8 words per ray at stride nrays */
+    uint32_t nrays;
 } trace_ctx;
+
+/* SimpleRayBuffer::store, src/Engine/RayBuffers.h:52-77: hitPoint = org +
+ * (t - EPS) * dir, dir, t, bestHit, each at stride numRays; dir is the
+ * direction the trace kernel restores, 1 / rcp (RayTracingKernels.h:128-130),
+ * and bestHit the traverser's reference index (OR_MISS on a miss, where the
+ * reference leaves it uninitialised) */
+static void store_raybuf(const trace_ctx *c, uint32_t i, const ray_state *r)
+{
+    float *b = c->raybuf + i;
+    const size_t s = c->nrays;
+    const float te = r->t - OR_EPS;
+    b[0] = r->o.x + te * r->D.x;
+    b[s] = r->o.y + te * r->D.y;
+    b[2 * s] = r->o.z + te * r->D.z;
+    b[3 * s] = r->D.x;
+    b[4 * s] = r->D.y;
+    b[5 * s] = r->D.z;
+    b[6 * s] = r->t;
+    memcpy(&b[7 * s], &r->best, 4);
+}
 
 /* occlusion threshold of the shadow traversers, TwoLevelGridTraverser.h:718,
  * SimpleGridTraverser.h:436 */
@@ -825,6 +847,7 @@ static void trace_uniform_one(const trace_ctx *c, uint32_t i)
     }
     c->prim[i] = r.best == OR_MISS ? OR_MISS : g->indices[r.best];
     c->t[i] = r.t;
+    if (c->raybuf) store_raybuf(c, i, &r);
     if (c->stats) {
         c->stats[4 * (size_t)i] = r.tests; c->stats[4 * (size_t)i + 1] = steps;
         c->stats[4 * (size_t)i + 2] = 0; c->stats[4 * (size_t)i + 3] = 0;
@@ -944,6 +967,7 @@ static void trace_twolevel_one(const trace_ctx *c, uint32_t i)
     }
     c->prim[i] = r.best == OR_MISS ? OR_MISS : g->indices[r.best];
     c->t[i] = r.t;
+    if (c->raybuf) store_raybuf(c, i, &r);
     if (c->stats) {
         c->stats[4 * (size_t)i] = r.tests; c->stats[4 * (size_t)i + 1] = lsteps;
         c->stats[4 * (size_t)i + 2] = tsteps; c->stats[4 * (size_t)i + 3] = entered;
@@ -963,14 +987,21 @@ static void trace_fn(void *c_, size_t b, size_t e, int tid)
 void or_trace(const or_grid *g, const float *const v[9], const float *const rays[7], uint32_t nrays,
               uint32_t *prim, float *t, uint32_t *stats, int nthreads)
 {
-    trace_ctx c = {g, v, rays, prim, t, stats, 0};
+    trace_ctx c = {g, v, rays, prim, t, stats, 0, NULL, nrays};
+    parallel_for(nrays, nthreads, trace_fn, &c);
+}
+
+void or_trace_store(const or_grid *g, const float *const v[9], const float *const rays[7], uint32_t nrays,
+                    uint32_t *prim, float *t, float *raybuf, int nthreads)
+{
+    trace_ctx c = {g, v, rays, prim, t, NULL, 0, raybuf, nrays};
     parallel_for(nrays, nthreads, trace_fn, &c);
 }
 
 void or_trace_shadow(const or_grid *g, const float *const v[9], const float *const rays[7], uint32_t nrays,
                      uint32_t *prim, float *t, uint32_t *stats, int nthreads)
 {
-    trace_ctx c = {g, v, rays, prim, t, stats, 1};
+    trace_ctx c = {g, v, rays, prim, t, stats, 1, NULL, nrays};
     parallel_for(nrays, nthreads, trace_fn, &c);
 }
 

@@ -110,6 +110,12 @@ void     or_grid_free(or_grid *g);
 void or_trace(const or_grid *g, const float *const v[9],
               const float *const rays[7], uint32_t nrays,
               uint32_t *prim, float *t, uint32_t *stats, int nthreads);
+/* or_trace that also writes SimpleRayBuffer::store's 8 words per ray
+ * (src/Engine/RayBuffers.h:52-77) into raybuf at stride nrays: hit point
+ * org + (t - EPS) dir, dir = 1 / (1 / d), t, bestHit (the reference index,
+ * OR_MISS on a miss) */
+void or_trace_store(const or_grid *g, const float *const v[9], const float *const rays[7], uint32_t nrays,
+                    uint32_t *prim, float *t, float *raybuf, int nthreads);
 /* any-hit (shadow) traversal of the same grid: the reference's shadow
  * traversers, which stop a ray once its t drops below 0.9999 (occluded) and
  * otherwise walk it to the grid exit; same outputs as or_trace. */

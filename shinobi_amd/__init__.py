@@ -380,6 +380,40 @@ class Grid:
                                          stats.ptr if stats is not None else None, self.ctx.stream), name)
         return prim, t
 
+    def device_view(self) -> L.GridDeviceView:
+        """device pointers of the grid and its scene (sg_grid_get_device_view)"""
+        v = L.GridDeviceView()
+        check(self.ctx._l.sg_grid_get_device_view(self.h, C.byref(v)), "sg_grid_get_device_view")
+        return v
+
+    def trace_store(self, rays: RayBatch, raybuf: DeviceBuffer | None = None, prim: DeviceBuffer | None = None,
+                    t: DeviceBuffer | None = None, d_count: C.c_void_p | None = None) -> DeviceBuffer:
+        """Closest hit with SimpleRayBuffer output (sg_trace_closest_store):
+        8 x n floats -- hit point xyz, dir xyz, t, bestHit bits (the reference
+        index, 0xFFFFFFFF on a miss); prim / t optional."""
+        n = rays.n
+        raybuf = raybuf or DeviceBuffer(self.ctx, 8 * 4 * max(n, 1))
+        r = rays.struct()
+        check(self.ctx._l.sg_trace_closest_store(self.ctx.h, self.h, C.byref(r), n, d_count, raybuf.ptr,
+                                                 prim.ptr if prim is not None else None,
+                                                 t.ptr if t is not None else None, self.ctx.stream),
+              "sg_trace_closest_store")
+        return raybuf
+
+    def trace_camera(self, cam: L.Pinhole, tile=(1, 1), prim: DeviceBuffer | None = None,
+                     t: DeviceBuffer | None = None, raybuf: DeviceBuffer | None = None,
+                     rays_out: RayBatch | None = None, stream=None) -> None:
+        """Primary rays generated on the device and traced tile by tile
+        (sg_trace_camera); every output is indexed by pixel id."""
+        tw, th = (tile, tile) if np.isscalar(tile) else tile
+        ro = rays_out.struct() if rays_out is not None else None
+        check(self.ctx._l.sg_trace_camera(self.ctx.h, self.h, C.byref(cam), tw, th,
+                                          prim.ptr if prim is not None else None,
+                                          t.ptr if t is not None else None,
+                                          raybuf.ptr if raybuf is not None else None,
+                                          C.byref(ro) if ro is not None else None,
+                                          self.ctx.stream if stream is None else stream), "sg_trace_camera")
+
     def trace_host(self, rays7: np.ndarray, with_stats: bool = False, shadow: bool = False):
         """Convenience: upload rays, trace, download (prim, t[, stats])."""
         rb = RayBatch(self.ctx, rays7)
@@ -723,6 +757,14 @@ def gen_cluster(seed: int, n_small: int = 500_000, n_large: int = 2_000) -> np.n
     a, p = _soa(n_small + n_large, 9)
     check(lib().sg_gen_cluster(seed, n_small, n_large, p), "sg_gen_cluster")
     return a
+
+
+def pinhole(eye, fwd, up, fov_deg: float, res_x: int, res_y: int) -> L.Pinhole:
+    """Camera::init (src/Primitive/Camera.h:41-60) for the device generator"""
+    c = L.Pinhole()
+    f3 = C.c_float * 3
+    check(lib().sg_pinhole_init(f3(*eye), f3(*fwd), f3(*up), fov_deg, res_x, res_y, C.byref(c)), "sg_pinhole_init")
+    return c
 
 
 def gen_camera_rays(eye, fwd, up, fov_deg: float, res_x: int, res_y: int) -> np.ndarray:

@@ -33,6 +33,7 @@ c_float_p = C.POINTER(C.c_float)
 c_u32_p = C.POINTER(C.c_uint32)
 Float9 = C.c_void_p * 9
 Float7 = C.c_void_p * 7
+Float3 = C.c_float * 3
 
 
 class BuildParams(C.Structure):
@@ -96,6 +97,18 @@ class Camera(C.Structure):
                 ("rotation", C.c_float * 3), ("res_x", C.c_uint32), ("res_y", C.c_uint32), ("fov_deg", C.c_float)]
 
 
+class Pinhole(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("top_left", C.c_float * 3), ("step_x", C.c_float * 3),
+                ("step_y", C.c_float * 3), ("res_x", C.c_uint32), ("res_y", C.c_uint32)]
+
+
+class GridDeviceView(C.Structure):
+    _fields_ = [("kind", C.c_int), ("status", C.c_int), ("res", C.c_int * 3), ("bmin", C.c_float * 3),
+                ("bmax", C.c_float * 3), ("cell_size", C.c_float * 3), ("cell_size_rcp", C.c_float * 3),
+                ("cells", C.c_void_p), ("leaves", C.c_void_p), ("n_cells", C.c_uint32), ("n_leaves", C.c_uint32),
+                ("indices", C.c_void_p), ("n_refs", C.c_uint32), ("tri", C.c_void_p * 9), ("n_tris", C.c_uint32)]
+
+
 class ServerStats(C.Structure):
     _fields_ = [("requests", C.c_uint64), ("rays", C.c_uint64), ("waves", C.c_uint64),
                 ("p50_us", C.c_double), ("p99_us", C.c_double), ("mean_us", C.c_double), ("max_us", C.c_double),
@@ -121,6 +134,7 @@ SIGNATURES = {
                                     C.c_int]),
     "sg_grid_get_info": (C.c_int, [C.c_void_p, C.POINTER(GridInfo)]),
     "sg_grid_download": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
+    "sg_grid_get_device_view": (C.c_int, [C.c_void_p, C.POINTER(GridDeviceView)]),
     "sg_grid_destroy": (C.c_int, [C.c_void_p]),
     "sg_trace_closest": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_uint32, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -128,6 +142,11 @@ SIGNATURES = {
                                   C.c_void_p, C.c_void_p, C.c_void_p]),
     "sg_bounce": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_uint32,
                             C.c_void_p, C.c_uint64, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sg_trace_closest_store": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_uint32, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sg_pinhole_init": (C.c_int, [Float3, Float3, Float3, C.c_float, C.c_uint32, C.c_uint32, C.POINTER(Pinhole)]),
+    "sg_trace_camera": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Pinhole), C.c_uint32, C.c_uint32, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_void_p]),
     "sg_bounce_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_uint32,
                                     C.c_void_p, C.c_uint64, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_void_p]),
     "sg_scan_u32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]),

@@ -536,6 +536,35 @@ sg_status sg_grid_get_info(sg_grid* g, sg_grid_info* out) {
     return SG_OK;
 }
 
+sg_status sg_grid_get_device_view(sg_grid* g, sg_grid_device_view* out) {
+    if (!g || !out) return SG_ERR_INVALID_ARG;
+    sg_grid_info info;
+    const sg_status r = sg_grid_get_info(g, &info);
+    if (r != SG_OK) return r;
+    const GridView v = grid_view(g);
+    memset(out, 0, sizeof(*out));
+    out->kind = g->kind;
+    out->status = info.status;
+    for (int i = 0; i < 3; ++i) {
+        out->res[i] = info.res[i];
+        out->bmin[i] = info.bmin[i];
+        out->bmax[i] = info.bmax[i];
+        out->cell_size[i] = info.cell_size[i];
+        out->cell_size_rcp[i] = info.cell_size_rcp[i];
+    }
+    const bool two = g->kind == SG_GRID_TWO_LEVEL;
+    out->cells = v.cells;
+    out->leaves = two ? v.leaves : nullptr;
+    out->n_cells = info.C1;
+    out->n_leaves = two ? info.C2 : 0u;
+    out->indices = v.indices;
+    out->n_refs = two ? info.P2 : info.P1;
+    const TriSoA t = scene_soa(g->scene);
+    for (int i = 0; i < 9; ++i) out->tri[i] = t.v[i];
+    out->n_tris = g->n_tris;
+    return SG_OK;
+}
+
 sg_status sg_grid_download(sg_grid* g, int which, void* host, size_t bytes) {
     if (!g || !host) return SG_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(g->ctx->device));
@@ -611,6 +640,66 @@ sg_status sg_trace_closest(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint32_
 sg_status sg_trace_shadow(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint32_t n, const uint32_t* d_count,
                           uint32_t* d_prim, float* d_t, uint32_t* d_stats, void* stream) {
     return trace_common(ctx, g, rays, n, d_count, d_prim, d_t, d_stats, true, stream);
+}
+
+sg_status sg_trace_closest_store(sg_ctx* ctx, sg_grid* g, const sg_rays* rays, uint32_t n, const uint32_t* d_count,
+                                 float* d_raybuf, uint32_t* d_prim, float* d_t, void* stream) {
+    if (!ctx || !g || !rays || !d_raybuf) return SG_ERR_INVALID_ARG;
+    for (int i = 0; i < 3; ++i)
+        if (!rays->o[i] || !rays->d[i]) return SG_ERR_INVALID_ARG;
+    if (!rays->tmax) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    clear_last_error();
+    TraceOut o{};
+    o.prim = d_prim;
+    o.t = d_t;
+    o.raybuf = d_raybuf;
+    o.stride = n;
+    launch_trace_gen(grid_view(g), ray_soa(rays), nullptr, n, d_count, o, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return SG_OK;
+}
+
+sg_status sg_trace_camera(sg_ctx* ctx, sg_grid* g, const sg_pinhole* cam, uint32_t tile_w, uint32_t tile_h,
+                          uint32_t* d_prim, float* d_t, float* d_raybuf, const sg_rays_out* rays_out, void* stream) {
+    if (!ctx || !g || !cam || cam->res_x == 0 || cam->res_y == 0) return SG_ERR_INVALID_ARG;
+    if (tile_w == 0 || tile_h == 0) return SG_ERR_INVALID_ARG;
+    const uint64_t n = (uint64_t)cam->res_x * cam->res_y;
+    if (n >= (1ull << 31)) return SG_ERR_INVALID_ARG;
+    if (rays_out) {
+        for (int i = 0; i < 3; ++i)
+            if (!rays_out->o[i] || !rays_out->d[i]) return SG_ERR_INVALID_ARG;
+        if (!rays_out->tmax) return SG_ERR_INVALID_ARG;
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    clear_last_error();
+    CamArgs c{};
+    for (int k = 0; k < 3; ++k) {
+        c.pos[k] = cam->position[k];
+        c.tl[k] = cam->top_left[k];
+        c.sx[k] = cam->step_x[k];
+        c.sy[k] = cam->step_y[k];
+    }
+    c.res_x = cam->res_x;
+    c.res_y = cam->res_y;
+    c.tw = std::min(tile_w, cam->res_x);
+    c.th = std::min(tile_h, cam->res_y);
+    TraceOut o{};
+    o.prim = d_prim;
+    o.t = d_t;
+    o.raybuf = d_raybuf;
+    o.stride = (uint32_t)n;
+    if (rays_out) {
+        for (int i = 0; i < 3; ++i) {
+            o.rays[i] = rays_out->o[i];
+            o.rays[3 + i] = rays_out->d[i];
+        }
+        o.rays[6] = rays_out->tmax;
+    }
+    RaySoA none{};
+    launch_trace_gen(grid_view(g), none, &c, (uint32_t)n, nullptr, o, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return SG_OK;
 }
 
 sg_status sg_bounce(sg_ctx* ctx, sg_scene* scene, const sg_rays* rays, const uint32_t* d_prim, const float* d_t,

@@ -165,14 +165,12 @@ sg_status sg_gen_cluster(uint64_t seed, uint32_t n_small, uint32_t n_large, floa
     return SG_OK;
 }
 
-// Camera::init + getDirection (src/Primitive/Camera.h:41-70) with the
-// RegularPixelSampler<1,1> of src/Primitive/PixelSampler.h:34-54 and
-// RegularPrimaryRayGenerator (src/Engine/RayGenerators.h:36-61); t = FLT_MAX
-sg_status sg_gen_camera_rays(const float eye[3], const float fwd[3], const float up[3], float fov_deg,
-                             uint32_t res_x, uint32_t res_y, float* const rays7[7]) {
-    for (int i = 0; i < 7; ++i)
-        if (!rays7[i]) return SG_ERR_INVALID_ARG;
-    if (res_x == 0 || res_y == 0) return SG_ERR_INVALID_ARG;
+// Camera::init (src/Primitive/Camera.h:41-60): normalised forward, right =
+// ~(forward x up), up = ~(forward x right); row / col vectors from the
+// vertical opening angle and the aspect ratio
+sg_status sg_pinhole_init(const float eye[3], const float fwd[3], const float up[3], float fov_deg, uint32_t res_x,
+                          uint32_t res_y, sg_pinhole* out) {
+    if (!eye || !fwd || !up || !out || res_x == 0 || res_y == 0) return SG_ERR_INVALID_ARG;
     const float aspect = (float)res_x / (float)res_y;
     const V f = norm(mk(fwd[0], fwd[1], fwd[2]));
     const V r = norm(cross(f, mk(up[0], up[1], up[2])));
@@ -183,6 +181,32 @@ sg_status sg_gen_camera_rays(const float eye[3], const float fwd[3], const float
     const V col = scl(scl(u, 2.f), tn);
     const V stepX = divs(row, (float)res_x), stepY = divs(col, (float)res_y);
     const V topLeft = sub(sub(f, divs(row, 2.f)), divs(col, 2.f));
+    const V e = mk(eye[0], eye[1], eye[2]);
+    const V* src[4] = {&e, &topLeft, &stepX, &stepY};
+    float* dst[4] = {out->position, out->top_left, out->step_x, out->step_y};
+    for (int k = 0; k < 4; ++k) {
+        dst[k][0] = src[k]->x;
+        dst[k][1] = src[k]->y;
+        dst[k][2] = src[k]->z;
+    }
+    out->res_x = res_x;
+    out->res_y = res_y;
+    return SG_OK;
+}
+
+// Camera::getDirection (src/Primitive/Camera.h:67-70) with the
+// RegularPixelSampler<1,1> of src/Primitive/PixelSampler.h:34-54 and
+// RegularPrimaryRayGenerator (src/Engine/RayGenerators.h:36-61); t = FLT_MAX
+sg_status sg_gen_camera_rays(const float eye[3], const float fwd[3], const float up[3], float fov_deg,
+                             uint32_t res_x, uint32_t res_y, float* const rays7[7]) {
+    for (int i = 0; i < 7; ++i)
+        if (!rays7[i]) return SG_ERR_INVALID_ARG;
+    sg_pinhole cam;
+    const sg_status st = sg_pinhole_init(eye, fwd, up, fov_deg, res_x, res_y, &cam);
+    if (st != SG_OK) return st;
+    const V topLeft = mk(cam.top_left[0], cam.top_left[1], cam.top_left[2]);
+    const V stepX = mk(cam.step_x[0], cam.step_x[1], cam.step_x[2]);
+    const V stepY = mk(cam.step_y[0], cam.step_y[1], cam.step_y[2]);
     const uint32_t n = res_x * res_y;
     const float rx = (float)res_x;
     parallel(n, [&](uint32_t b, uint32_t e) {

@@ -173,6 +173,41 @@ void launch_trace_items(const TraceItem* items, const ItemSplit& split, bool sha
 // (block: threads per block, 64, 128 or 256)
 void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t* d_n, uint32_t* prim,
                   float* t, uint32_t* stats, bool shadow, int block, hipStream_t st);
+
+// Primary rays generated inside the trace kernel, as RegularPrimaryRayGenerator
+// (src/Engine/RayGenerators.h:36-61) does per ray id: the pinhole Camera after
+// init (src/Primitive/Camera.h:41-70) and RegularPixelSampler<1,1>
+// (src/Primitive/PixelSampler.h:34-54); t starts at FLT_MAX.  The launch
+// visits the pixels tile by tile (tiles of tw x th pixels in row-major order,
+// row-major inside a tile, partial tiles at the right and bottom edges; 1 x 1
+// is the reference's row-major order) and stores every output at the pixel
+// id, so the order changes only which rays share a wave.
+struct CamArgs {
+    float pos[3], tl[3], sx[3], sy[3];   // position, top-left direction, pixel steps
+    uint32_t res_x, res_y, tw, th;
+};
+SG_HD uint32_t tile_pixel(const CamArgs& c, uint32_t k) {
+    const uint32_t band = c.th * c.res_x;            // pixels of one full row of tiles
+    const uint32_t b = k / band, r = k - b * band;
+    const uint32_t hb = c.res_y - b * c.th < c.th ? c.res_y - b * c.th : c.th;
+    const uint32_t tsz = c.tw * hb;                  // pixels of a full-width tile of this row
+    const uint32_t j = r / tsz, s = r - j * tsz;
+    const uint32_t wj = c.res_x - j * c.tw < c.tw ? c.res_x - j * c.tw : c.tw;
+    return (b * c.th + s / wj) * c.res_x + j * c.tw + s % wj;
+}
+// Outputs of one trace launch; every pointer is optional (null: not written).
+struct TraceOut {
+    uint32_t* prim;     // triangle id, 0xFFFFFFFF on a miss
+    float* t;           // final t
+    float* raybuf;      // SimpleRayBuffer::store (src/Engine/RayBuffers.h:52-77): 8 words per ray
+                        // at stride `stride`: hit point org + (t - EPS) dir, dir, t, bestHit
+    uint32_t stride;
+    float* rays[7];     // camera launches: the generated rays (SoA, at the pixel id)
+};
+// closest hit of SoA rays (cam == nullptr) or of camera rays (n = res_x *
+// res_y, d_n ignored) with the outputs of `out`
+void launch_trace_gen(const GridView& g, const RaySoA& r, const CamArgs* cam, uint32_t n, const uint32_t* d_n,
+                      const TraceOut& out, hipStream_t st);
 // one cosine-weighted bounce per hit, compacted and ordered by direction
 // class in rounds of 256 rays (input order within a class); *d_count
 // receives the number of bounce rays.  scratch: bounce_scratch_bytes(n).

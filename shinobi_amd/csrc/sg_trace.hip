@@ -534,6 +534,35 @@ SG_D void walk_twolevel(const GView& g, const GParams& p, Ray& r, Walk& w, uint3
     }
 }
 
+// A ray's start: rcp(dir) as the trace kernel computes it
+// (src/Engine/RayTracingKernels.h:111-113) and the intersector's recomputed
+// direction D = 1/rcp (src/Intersector/SimpleIntersector.hpp:58)
+SG_D void ray_begin(Ray& r, v3 o, v3 d, float t) {
+    r.o = o;
+    r.rcp = mk(rcp_rn(d.x), rcp_rn(d.y), rcp_rn(d.z));
+    r.D = mk(rcp_rn(r.rcp.x), rcp_rn(r.rcp.y), rcp_rn(r.rcp.z));
+    r.fast = div_fast_range(r.rcp.x) && div_fast_range(r.rcp.y) && div_fast_range(r.rcp.z);
+    r.t = t;
+    r.best = kMiss;
+    r.tests = 0;
+}
+
+// traverse (both traversers): leaves r.t and r.best (a reference index)
+template <int KIND, bool SHADOW>
+SG_D void ray_walk(const GView& g, const GParams& p, Ray& r, uint32_t& s1, uint32_t& s2, uint32_t& s3) {
+    Walk w;
+    if (KIND == 0) {
+        walk_init_uniform<SHADOW>(p, r, w);
+        walk_uniform<SHADOW>(g, p, r, w, s1);
+    } else {
+        walk_init_twolevel<SHADOW>(p, r, w);
+        walk_twolevel<SHADOW>(g, p, r, w, s1, s2, s3);
+    }
+}
+
+// a failed or overflowed build is never traversed: every ray misses
+SG_D bool grid_failed(const GView& g) { return (g.hdr->status & ~(ST_LEAF_RES | ST_NEG_EXTENT)) != 0u; }
+
 // the trace kernel of src/Engine/RayTracingKernels.h:37-136 for one SoA ray:
 // rcp(dir) (:111-113), traverse, store prim = indices[best] (miss: ~0u) and t
 // SPARSE: lanes without a ray (valid false) run the walk loop inactive, so
@@ -541,8 +570,7 @@ SG_D void walk_twolevel(const GView& g, const GParams& p, Ray& r, Walk& w, uint3
 template <int KIND, bool STATS, bool SHADOW, bool SPARSE = false>
 SG_D void trace_one(const GView& g, const GRays& rays, uint32_t i, gptr<uint32_t> prim, gptr<float> tout,
                     gptr<uint32_t> stats, bool valid = true) {
-    // a failed or overflowed build is never traversed: every ray misses
-    if (g.hdr->status & ~(ST_LEAF_RES | ST_NEG_EXTENT)) {
+    if (grid_failed(g)) {
         if (!SPARSE || valid) {
             prim[i] = kMiss;
             tout[i] = rays.a[6][i];
@@ -554,23 +582,10 @@ SG_D void trace_one(const GView& g, const GRays& rays, uint32_t i, gptr<uint32_t
     const GParams p = load_gparams(g.hdr);
     const uint32_t j = SPARSE && !valid ? 0u : i;   // a lane without a ray reads ray 0, inactive
     Ray r;
-    r.o = mk(rays.a[0][j], rays.a[1][j], rays.a[2][j]);
-    const v3 d = mk(rays.a[3][j], rays.a[4][j], rays.a[5][j]);
-    r.rcp = mk(rcp_rn(d.x), rcp_rn(d.y), rcp_rn(d.z));
-    r.D = mk(rcp_rn(r.rcp.x), rcp_rn(r.rcp.y), rcp_rn(r.rcp.z));
-    r.fast = div_fast_range(r.rcp.x) && div_fast_range(r.rcp.y) && div_fast_range(r.rcp.z);
-    r.t = SPARSE && !valid ? 0.f : rays.a[6][j];   // t = 0: never traversed
-    r.best = kMiss;
-    r.tests = 0;
+    ray_begin(r, mk(rays.a[0][j], rays.a[1][j], rays.a[2][j]), mk(rays.a[3][j], rays.a[4][j], rays.a[5][j]),
+              SPARSE && !valid ? 0.f : rays.a[6][j]);   // t = 0: never traversed
     uint32_t s1 = 0, s2 = 0, s3 = 0;
-    Walk w;
-    if (KIND == 0) {
-        walk_init_uniform<SHADOW>(p, r, w);
-        walk_uniform<SHADOW>(g, p, r, w, s1);
-    } else {
-        walk_init_twolevel<SHADOW>(p, r, w);
-        walk_twolevel<SHADOW>(g, p, r, w, s1, s2, s3);
-    }
+    ray_walk<KIND, SHADOW>(g, p, r, s1, s2, s3);
     if (SPARSE && !valid) return;
     prim[i] = r.best == kMiss ? kMiss : g.indices[r.best];
     tout[i] = r.t;
@@ -673,6 +688,107 @@ void launch_trace(const GridView& g, const RaySoA& r, uint32_t n, const uint32_t
         else { if (stats) SG_TRACE(1, true, false); else SG_TRACE(1, false, false); }
     }
 #undef SG_TRACE
+}
+
+// ------------------------------------ camera rays and SimpleRayBuffer output
+// RegularPrimaryRayGenerator<..>::operator() (src/Engine/RayGenerators.h:40-58)
+// for pixel id `id`: RegularPixelSampler<1,1>::getPixelCoords with sample 0
+// (src/Primitive/PixelSampler.h:37-52), then Camera::getDirection
+// (src/Primitive/Camera.h:67-70), topLeft + x stepX + y stepY
+SG_D v3 camera_dir(const CamArgs& c, uint32_t id) {
+    float py = (truncf(0.f) + 0.5f) / 1.f;
+    float px = 0.f - truncf(0.f) + 0.5f / 1.f;
+    const float rx = (float)c.res_x;
+    const float tmp = (float)id / rx;
+    py += truncf(tmp);
+    px += (tmp - truncf(tmp)) * rx;
+    return (mk(c.tl[0], c.tl[1], c.tl[2]) + scl(mk(c.sx[0], c.sx[1], c.sx[2]), px)) +
+           scl(mk(c.sy[0], c.sy[1], c.sy[2]), py);
+}
+
+// Camera launches write their outputs at pixel ids, so the waves of
+// neighbouring tiles share output cache lines (2-pixel-wide tiles: 8 waves
+// per 64-byte line of each output array).  Blocks are dispatched to the 8
+// XCDs round-robin and each XCD has its own L2, so neighbouring blocks
+// would each write back a partial line.  This remaps the block index so
+// that 8 consecutive tiles run on one XCD: block b = 8 j + x runs on XCD x,
+// and takes the (j mod 8)-th tile of group 8 (j / 8) + x.  Blocks beyond the
+// last whole 64 keep their index.
+constexpr uint32_t kXcds = 8;
+SG_D uint32_t xcd_group_block(uint32_t b, uint32_t nb) {
+    if (b >= (nb & ~(kXcds * kXcds - 1u))) return b;
+    const uint32_t x = b % kXcds, j = b / kXcds;
+    return ((j / kXcds) * kXcds + x) * kXcds + j % kXcds;
+}
+
+// Closest hit with the reference trace kernel's full contract: rays from SoA
+// arrays (CAM = false, output slot = ray index) or generated from the camera
+// (CAM = true, ray k of the launch is pixel tile_pixel(k), output slot = the
+// pixel id); outputs prim / t and SimpleRayBuffer::store
+// (src/Engine/RayBuffers.h:52-77) with the direction the trace kernel
+// restores, 1 / rcp (src/Engine/RayTracingKernels.h:128-130), and bestHit =
+// the reference index the traverser returns (0xFFFFFFFF on a miss).
+template <int KIND, bool CAM>
+__global__ __launch_bounds__(kBlock) SG_TRACE_OCCUPANCY void k_trace_gen(GridView g, RaySoA rays, CamArgs cam,
+                                                                        uint32_t cap, const uint32_t* d_n,
+                                                                        TraceOut out) {
+    const uint32_t n = (!CAM && d_n) ? min(*d_n, cap) : cap;
+    const uint32_t i = (CAM ? xcd_group_block(blockIdx.x, gridDim.x) : blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const GView gv = gview(g);
+    const bool failed = grid_failed(gv);
+    const GParams p = load_gparams(gv.hdr);
+    uint32_t slot = i;
+    Ray r;
+    if (CAM) {
+        slot = tile_pixel(cam, i);
+        const v3 o = mk(cam.pos[0], cam.pos[1], cam.pos[2]);
+        const v3 d = camera_dir(cam, slot);
+        if (out.rays[0]) {
+            const float v[7] = {o.x, o.y, o.z, d.x, d.y, d.z, 3.402823466e+38f};
+            for (int k = 0; k < 7; ++k) out.rays[k][slot] = v[k];
+        }
+        ray_begin(r, o, d, 3.402823466e+38f);
+    } else {
+        const GRays gr = grays(rays);
+        ray_begin(r, mk(gr.a[0][i], gr.a[1][i], gr.a[2][i]), mk(gr.a[3][i], gr.a[4][i], gr.a[5][i]), gr.a[6][i]);
+    }
+    if (!failed) {
+        uint32_t s1 = 0, s2 = 0, s3 = 0;
+        ray_walk<KIND, false>(gv, p, r, s1, s2, s3);
+    }
+    if (out.prim) out.prim[slot] = r.best == kMiss ? kMiss : gv.indices[r.best];
+    if (out.t) out.t[slot] = r.t;
+    if (out.raybuf) {
+        // hitPoint = org + (t - EPS) * dir, then dir, t and bestHit, stride apart
+        const float te = r.t - kEps;
+        const size_t s = out.stride;
+        float* b = out.raybuf + slot;
+        b[0] = r.o.x + te * r.D.x;
+        b[s] = r.o.y + te * r.D.y;
+        b[2 * s] = r.o.z + te * r.D.z;
+        b[3 * s] = r.D.x;
+        b[4 * s] = r.D.y;
+        b[5 * s] = r.D.z;
+        b[6 * s] = r.t;
+        b[7 * s] = __uint_as_float(r.best);
+    }
+}
+
+void launch_trace_gen(const GridView& g, const RaySoA& r, const CamArgs* cam, uint32_t n, const uint32_t* d_n,
+                      const TraceOut& out, hipStream_t st) {
+    if (n == 0) return;
+    constexpr int B = 64;   // one-wave blocks, as launch_trace
+    const dim3 grid((n + B - 1) / B), tb(B);
+    CamArgs c{};
+    if (cam) c = *cam;
+    if (g.kind == 0) {
+        if (cam) hipLaunchKernelGGL((k_trace_gen<0, true>), grid, tb, 0, st, g, r, c, n, d_n, out);
+        else hipLaunchKernelGGL((k_trace_gen<0, false>), grid, tb, 0, st, g, r, c, n, d_n, out);
+    } else {
+        if (cam) hipLaunchKernelGGL((k_trace_gen<1, true>), grid, tb, 0, st, g, r, c, n, d_n, out);
+        else hipLaunchKernelGGL((k_trace_gen<1, false>), grid, tb, 0, st, g, r, c, n, d_n, out);
+    }
 }
 
 // ------------------------------------------------- many batches, one launch

@@ -40,20 +40,29 @@ class Workload:
     bounce_seed: int | None = None
     # oracle keyword arguments for the same build
     oracle: dict = field(default_factory=dict)
+    # the pinhole camera of the primary rays: (eye, forward, up, fov_deg,
+    # res_x, res_y), for the device generator (shinobi_amd.pinhole)
+    camera: tuple | None = None
 
 
 def soup10k(n: int = 10_000, res: int = 256, seed: int = 1) -> Workload:
     scene = gen_soup(seed, n, 0.01)
-    rays = gen_camera_rays((0.5, 0.5, -1.0), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0, res, res)
+    cam = ((0.5, 0.5, -1.0), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0, res, res)
+    rays = gen_camera_rays(*cam)
     return Workload("soup10k", SG_GRID_UNIFORM, scene, rays, dict(lam=5.0),
-                    oracle=dict(lam=5.0, top_divisor=1))
+                    oracle=dict(lam=5.0, top_divisor=1), camera=cam)
 
 
-def heightfield_camera(seed: int, res_x: int, res_y: int, amplitude: float = 0.2) -> np.ndarray:
+def heightfield_pinhole(seed: int, res_x: int, res_y: int, amplitude: float = 0.2) -> tuple:
+    """camera of the height field: from (0.5,-0.3,0.6) toward the field centre"""
     hc = heightfield_height(seed, 0.5, 0.5, amplitude)
     eye = (0.5, -0.3, 0.6)
     fwd = (0.5 - eye[0], 0.5 - eye[1], hc - eye[2])
-    return gen_camera_rays(eye, fwd, (0.0, 0.0, 1.0), 60.0, res_x, res_y)
+    return (eye, fwd, (0.0, 0.0, 1.0), 60.0, res_x, res_y)
+
+
+def heightfield_camera(seed: int, res_x: int, res_y: int, amplitude: float = 0.2) -> np.ndarray:
+    return gen_camera_rays(*heightfield_pinhole(seed, res_x, res_y, amplitude))
 
 
 def tile_order(res_x: int, res_y: int, tile) -> np.ndarray:
@@ -80,17 +89,18 @@ def heightfield2m(grid: int = 1024, res_x: int = 1920, res_y: int = 1080, seed: 
         rays = np.ascontiguousarray(rays[:, tile_order(res_x, res_y, tile)])
     return Workload("heightfield2m", SG_GRID_TWO_LEVEL, scene, rays,
                     dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16), bounce_seed=seed + 100,
-                    oracle=dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16))
+                    oracle=dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16),
+                    camera=heightfield_pinhole(seed, res_x, res_y))
 
 
 def cluster(n_small: int = 500_000, n_large: int = 2_000, res_x: int = 1280, res_y: int = 720,
             seed: int = 3) -> Workload:
     scene = gen_cluster(seed, n_small, n_large)
-    eye = (0.5, 0.5, -0.6)
-    rays = gen_camera_rays(eye, (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0, res_x, res_y)
+    cam = ((0.5, 0.5, -0.6), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 60.0, res_x, res_y)
+    rays = gen_camera_rays(*cam)
     return Workload("cluster", SG_GRID_TWO_LEVEL, scene, rays,
                     dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16), bounce_seed=seed + 100,
-                    oracle=dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16))
+                    oracle=dict(lam=5.0, top_divisor=6, lambda_leaf=2.0, leaf_min_refs=16), camera=cam)
 
 
 def sphere_mesh(n_tris: int = 200_000, frame: int = 0, seed: int = 5) -> np.ndarray:
@@ -130,9 +140,11 @@ def scene_file(obj_path: str, camera_path: str, name: str = "ruins", tile=1) -> 
     else:
         scene, _ = load_obj(obj_path)
     cam = load_camera(camera_path)
-    rays = gen_camera_rays(cam["position"], cam["orientation"], cam["up"], cam["fov_deg"], cam["res_x"], cam["res_y"])
+    pin = (tuple(cam["position"]), tuple(cam["orientation"]), tuple(cam["up"]), cam["fov_deg"], cam["res_x"],
+           cam["res_y"])
+    rays = gen_camera_rays(*pin)
     if tile != 1:
         rays = np.ascontiguousarray(rays[:, tile_order(cam["res_x"], cam["res_y"], tile)])
     return Workload(name, SG_GRID_TWO_LEVEL, scene, rays,
                     dict(lam=5.0, top_divisor=6, lambda_leaf=1.2, leaf_min_refs=16), bounce_seed=107,
-                    oracle=dict(lam=5.0, top_divisor=6, lambda_leaf=1.2, leaf_min_refs=16))
+                    oracle=dict(lam=5.0, top_divisor=6, lambda_leaf=1.2, leaf_min_refs=16), camera=pin)

@@ -227,3 +227,35 @@ def test_shadow_any_hit(oracle, scene, rays, kind):
     short[6] = 0.5
     ps, ts = g.trace(short, shadow=True)
     assert (ps == MISS).all() and (ts == 0.5).all()
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_trace_store_geometry(oracle, scene, rays, kind):
+    """SimpleRayBuffer::store (src/Engine/RayBuffers.h:52-77) pinned by
+    geometry in float64, not by its own formula: the stored point plus
+    EPS * dir lies on the plane of the hit triangle, the point sits EPS * dir
+    before the hit (a dropped or flipped EPS moves it by 1e-5 |dir|), the
+    stored direction is the ray's to within 1 ulp, t equals the trace's, and
+    bestHit maps to the hit triangle through indices (misses: 0xFFFFFFFF)."""
+    g = (oracle.build_twolevel(scene, lambda_leaf=2.0) if kind else oracle.build_uniform(scene, top_divisor=1))
+    prim, t = g.trace(rays)
+    buf = g.trace_store(rays)
+    hit = prim != MISS
+    assert hit.sum() > 100 and (~hit).sum() > 100
+    np.testing.assert_array_equal(buf[6].view(np.uint32), t.view(np.uint32))
+    best = buf[7].view(np.uint32)
+    assert (best[~hit] == MISS).all()
+    np.testing.assert_array_equal(g.indices[best[hit]], prim[hit])
+    d = rays[3:6].astype(np.float64)
+    D = buf[3:6].astype(np.float64)
+    assert np.all(np.abs(D - d) <= 1.2e-7 * np.abs(d) + 1e-30)
+    o = rays[0:3, hit].astype(np.float64)
+    th = t[hit].astype(np.float64)
+    P = buf[0:3, hit].astype(np.float64)
+    onhit = o + th * D[:, hit]
+    np.testing.assert_allclose(P, onhit - 1e-5 * D[:, hit], rtol=0, atol=2e-6)
+    v = scene.astype(np.float64).reshape(3, 3, -1)[:, :, prim[hit]]
+    nrm = np.cross(v[1] - v[0], v[2] - v[0], axis=0)
+    nrm /= np.linalg.norm(nrm, axis=0)
+    plane = np.abs(np.sum((P + 1e-5 * D[:, hit] - v[0]) * nrm, axis=0))
+    assert np.max(plane) < 2e-6
