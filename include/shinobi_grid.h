@@ -424,6 +424,11 @@ sg_status sg_server_submit(sg_server *server, uint32_t scene_id, const float *co
  * last build failed or outgrew the buffers (for example a caller's rebuild
  * of a grid added with sg_server_add_grid). */
 sg_status sg_server_wait(sg_server *server, uint64_t request_id, double *latency_us);
+/* sg_server_wait that also reports which mesh of the request's scene it was
+ * traced against: 0 for a static scene and for the mesh a dynamic scene was
+ * added with, k for the mesh of the k-th successful sg_server_update_dynamic
+ * of that scene.  No counterpart in the reference (one mesh per frame). */
+sg_status sg_server_wait_version(sg_server *server, uint64_t request_id, double *latency_us, uint64_t *version);
 /* wait for every submitted request */
 sg_status sg_server_drain(sg_server *server);
 sg_status sg_server_get_stats(sg_server *server, sg_server_stats *out);

@@ -696,13 +696,15 @@ class Server:
         self._keep[int(rid.value)] = (rays7, prim, t)
         return int(rid.value)
 
-    def wait(self, rid: int):
-        """-> (prim, t, latency_us)"""
+    def wait(self, rid: int, with_version: bool = False):
+        """-> (prim, t, latency_us), or with_version (prim, t, latency_us,
+        mesh version of the request's scene it ran on)"""
         lat = C.c_double()
-        st = self.ctx._l.sg_server_wait(self.h, rid, C.byref(lat))
+        ver = C.c_uint64()
+        st = self.ctx._l.sg_server_wait_version(self.h, rid, C.byref(lat), C.byref(ver))
         _, prim, t = self._keep.pop(rid)
         check(st, "sg_server_wait")
-        return prim, t, float(lat.value)
+        return (prim, t, float(lat.value), int(ver.value)) if with_version else (prim, t, float(lat.value))
 
     def drain(self) -> None:
         check(self.ctx._l.sg_server_drain(self.h), "sg_server_drain")

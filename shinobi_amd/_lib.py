@@ -186,6 +186,7 @@ SIGNATURES = {
     "sg_server_submit": (C.c_int, [C.c_void_p, C.c_uint32, Float7, C.c_uint32, C.c_void_p, C.c_void_p,
                                    C.POINTER(C.c_uint64)]),
     "sg_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double)]),
+    "sg_server_wait_version": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "sg_server_drain": (C.c_int, [C.c_void_p]),
     "sg_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "sg_server_reset_stats": (C.c_int, [C.c_void_p]),

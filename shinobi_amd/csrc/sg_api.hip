@@ -349,14 +349,23 @@ static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st, void* cons
     if (!two) return;
     launch_leaf_cells(g->hdr, a, g->ranges1, g->cells1, g->dbg_leafc, g->sums_cells, st);
     mark(5);
-    launch_leaf_emit(t, g->hdr, a, g->k1[f1], g->v1[f1], g->ranges1, g->cells1, g->k2[0], g->v2[0], g->refs,
-                     g->dbg_pair, g->sums_leaf, g->leaf_info, g->large_path ? &g->large : nullptr, st);
+    launch_leaf_emit(t, g->hdr, a, g->k1[f1], g->v1[f1], g->ranges1, g->cells1, g->k2[0], g->v2[0], g->k2[1],
+                     g->v2[1], g->refs, g->dbg_pair, g->sums_leaf, g->leaf_info, g->large_path ? &g->large : nullptr,
+                     st);
     mark(6);
-    launch_leaf_sort_fixup(g->k2[0], g->v2[0], g->k2[1], g->v2[1], g->hdr, g->cap_P2, g->key_bits2, g->rb, g->sort2,
-                           st);
+    // Only a grid whose build met leaf tiles too large for the LDS sort has
+    // the parallel emission of such tiles and needs the global sort after
+    // it; any other grid sorts every tile inside the emission (an oversize
+    // tile that a rebuild meets is sorted by its block), so its rebuilds
+    // enqueue neither the sort passes nor the record pass behind them.
+    if (g->large_path)
+        launch_leaf_sort_fixup(g->k2[0], g->v2[0], g->k2[1], g->v2[1], g->hdr, g->cap_P2, g->key_bits2, g->rb,
+                               g->sort2, st);
     mark(7);
     launch_ranges(g->k2[0], &g->hdr->P2, g->cap_P2, &g->hdr->C2, g->leaves, &g->hdr->P2_valid, st);
-    launch_ref_records(g->v2[0], &g->hdr->P2_valid, &g->hdr->leaf_sort, g->cap_P2, g->scene->rec, N, g->refs, st);
+    if (g->large_path)
+        launch_ref_records(g->v2[0], &g->hdr->P2_valid, &g->hdr->leaf_sort, g->cap_P2, g->scene->rec, N, g->refs,
+                           st);
     mark(8);
 }
 
@@ -460,7 +469,7 @@ sg_status sg_build(sg_ctx* ctx, sg_scene* scene, const sg_build_params* p, void*
             if (two && (double)g->cap_P2 < h.need_P2 * (double)slack) { g->cap_P2 = grow(h.need_P2); again = true; }
             // a tile too large for LDS: lay out the parallel emission and
             // build again with it (rebuilds then use it too)
-            if (two && h.leaf_sort > 0 && !g->large_path) { g->large_path = true; again = true; }
+            if (two && h.oversize > 0 && !g->large_path) { g->large_path = true; again = true; }
             if (!again) {
                 *out = g;
                 return SG_OK;

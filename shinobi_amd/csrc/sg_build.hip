@@ -30,6 +30,7 @@ __global__ void k_init(GridHeader* h, uint32_t n_tris, ZeroSegs segs) {
         h->need_P1 = h->need_C2 = h->need_P2 = 0;
         h->status = 0;
         h->leaf_sort = 0;
+        h->oversize = 0;
     }
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (int s = 0; s < segs.count; ++s) {
@@ -48,48 +49,21 @@ SG_D void tri_load(const TriSoA& t, uint32_t i, v3& a, v3& b, v3& c) {
 
 // --------------------------------------------------------------- bounds
 // Scene AABB (the host loop of FastTwoLevelGridBuilder.h:75-81) as a device
-// reduction; exact because min/max never round.  The same pass writes the
-// per-triangle records (TriSoA::rec), so they always match the vertices the
-// grid is built from: a block takes 256 triangles (coalesced SoA loads, one
-// triangle per lane), stages their 4 KB x 4 of records in LDS and stores
-// them with lane-consecutive 16-byte writes (full lines, no partial-line
-// write traffic).  Each block leaves its six partial keys in `part`;
-// k_params folds them (no same-address atomics).
+// reduction; exact because min/max never round.  Coalesced SoA loads, one
+// triangle per lane.  The per-triangle records (TriSoA::rec) are written
+// when the vertices change (sg_scene_create / sg_scene_update, k_prep_tris),
+// not here.  Each block leaves its six partial keys in `part`; k_params
+// folds them (no same-address atomics).
 __global__ __launch_bounds__(kBlock) void k_bounds(TriSoA t, uint32_t n, uint32_t* __restrict__ part) {
     __shared__ uint32_t s_red[6][kBlock / 64];
-    __shared__ float4 s_rec[4 * kBlock];
     uint32_t k[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-    const uint32_t chunks = (n + kBlock - 1) / kBlock;
-    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) {
-        const uint32_t base = c * kBlock;
-        const uint32_t i = base + threadIdx.x;
-        if (i < n) {
-            float w[9];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
 #pragma unroll
-            for (int q = 0; q < 9; ++q) w[q] = t.v[q][i];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const uint32_t k0 = fkey(w[a]), k1 = fkey(w[3 + a]), k2 = fkey(w[6 + a]);
-                k[a] = min(k[a], min(k0, min(k1, k2)));
-                k[3 + a] = max(k[3 + a], max(k0, max(k1, k2)));
-            }
-            const v3 v0 = mk(w[0], w[1], w[2]), v1 = mk(w[3], w[4], w[5]), v2 = mk(w[6], w[7], w[8]);
-            const v3 e1 = v1 - v0, e2 = v2 - v0;
-            // record component j of triangle threadIdx.x at s_rec[j * kBlock + threadIdx.x]
-            s_rec[0 * kBlock + threadIdx.x] = make_float4(v0.x, v0.y, v0.z, e1.x);
-            s_rec[1 * kBlock + threadIdx.x] = make_float4(e1.y, e1.z, e2.x, e2.y);
-            s_rec[2 * kBlock + threadIdx.x] = make_float4(e2.z, v1.x, v1.y, v1.z);
-            s_rec[3 * kBlock + threadIdx.x] = make_float4(v2.x, v2.y, v2.z, 0.f);
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t k0 = fkey(t.v[a][i]), k1 = fkey(t.v[3 + a][i]), k2 = fkey(t.v[6 + a][i]);
+            k[a] = min(k[a], min(k0, min(k1, k2)));
+            k[3 + a] = max(k[3 + a], max(k0, max(k1, k2)));
         }
-        __syncthreads();
-        // float4 number f = 4 * tri + component of the block's 1024 float4s
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint32_t f = (uint32_t)j * kBlock + threadIdx.x;
-            const uint32_t tri = f >> 2;
-            if (base + tri < n) t.rec[4 * (size_t)base + f] = s_rec[(f & 3u) * kBlock + tri];
-        }
-        __syncthreads();
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
@@ -910,8 +884,70 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* 
     }
 }
 
-// tiles too large for LDS: pairs in emission order, sorted afterwards by the
-// global pass (GridHeader::leaf_sort receives the number of accepted pairs)
+// One stable LSD pass by a block of kBlock threads over n (key, value)
+// pairs in global memory: digit ((k - kbase) >> shift) & (2^db - 1), db <= 8,
+// (k, v) -> (ko, vo).  Ranks as lds_sort_pass: a digit histogram, its scan,
+// then rounds of kBlock pairs ranked by wave ballots and ordered across
+// waves by per-wave digit counts.
+static_assert(kBlock == 256, "one digit bin per thread");
+SG_D void block_sort_pass_global(const uint32_t* k, const uint32_t* v, uint32_t* ko, uint32_t* vo, uint32_t n,
+                                 uint32_t kbase, int shift, int db, uint32_t* s_base, uint32_t* s_wc,
+                                 uint32_t* s_scan) {
+    constexpr int NW = kBlock / 64;
+    constexpr int RM = 256;
+    const uint32_t R = 1u << db, mask = R - 1u;
+    const int w = wave_id();
+    const uint64_t lt = lanemask_lt();
+    for (uint32_t j = threadIdx.x; j < RM * NW; j += kBlock) s_wc[j] = 0;
+    s_base[threadIdx.x] = 0;   // kBlock == RM
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += kBlock) atomicAdd(&s_base[((k[i] - kbase) >> shift) & mask], 1u);
+    __syncthreads();
+    {
+        const uint32_t c = threadIdx.x < R ? s_base[threadIdx.x] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<NW>(c, &tot, s_scan);
+        if (threadIdx.x < R) s_base[threadIdx.x] = ex;
+        __syncthreads();
+    }
+    for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
+        const uint32_t i = r0 + threadIdx.x;
+        const bool ok = i < n;
+        const uint32_t x = ok ? k[i] : 0u, y = ok ? v[i] : 0u;
+        const uint32_t d = ((x - kbase) >> shift) & mask;
+        const uint64_t peers = digit_peers(ok, d, db);
+        const uint32_t rk = (uint32_t)__popcll(peers & lt);
+        if (ok && rk == 0) s_wc[w * RM + d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (ok) {
+            uint32_t pos = s_base[d] + rk;
+            for (int u = 0; u < w; ++u) pos += s_wc[u * RM + d];
+            ko[pos] = x;
+            vo[pos] = y;
+        }
+        __syncthreads();
+        if (threadIdx.x < R) {
+            uint32_t c = 0;
+#pragma unroll
+            for (int u = 0; u < NW; ++u) {
+                c += s_wc[u * RM + threadIdx.x];
+                s_wc[u * RM + threadIdx.x] = 0;
+            }
+            s_base[threadIdx.x] += c;
+        }
+        __syncthreads();
+    }
+}
+
+// Tiles too large for LDS (more than kLeafStage pairs, or a local key range
+// or top-pair count beyond the LDS sort's bit budget): one block per tile
+// emits the tile's pairs in emission order.  With self_sort, the grid has no
+// global leaf sort scheduled, so the block then sorts its accepted pairs
+// itself (stable LSD passes over the tile-local key, ping-ponging with
+// tk / tv) and writes their reference records -- correct but slow, for a
+// rebuild that meets an oversize tile the grid's build never met.  Without
+// self_sort the global pass sorts them (GridHeader::leaf_sort receives the
+// number of accepted pairs).  Either way GridHeader::oversize counts the tile.
 __global__ __launch_bounds__(kBlock) void k_leaf_emit_large(TriSoA t, GridHeader* h, BuildArgs a,
                                                             const uint32_t* __restrict__ top_keys,
                                                             const uint32_t* __restrict__ top_vals,
@@ -919,14 +955,20 @@ __global__ __launch_bounds__(kBlock) void k_leaf_emit_large(TriSoA t, GridHeader
                                                             uint32_t* __restrict__ keys,
                                                             uint32_t* __restrict__ vals,
                                                             const uint32_t* __restrict__ sums,
-                                                            const uint32_t* __restrict__ info, uint32_t T) {
+                                                            const uint32_t* __restrict__ info, uint32_t T,
+                                                            int self_sort, uint32_t* __restrict__ tk,
+                                                            uint32_t* __restrict__ tv, float* __restrict__ refs) {
     __shared__ uint32_t s_scan[kBlock / 64];
+    __shared__ uint32_t s_base[256], s_wc[256 * (kBlock / 64)];
     LeafTilePlan q;
     if (!leaf_tile_plan(a, sums, info, T, blockIdx.x, q) || q.staged) return;
     const LeafParams L = read_leaf_params(h);
     const uint32_t cap = a.cap_P2;
     const uint32_t rej_base = q.tot_acc + q.pre_rej;
-    if (threadIdx.x == 0) h->leaf_sort = q.tot_acc;
+    if (threadIdx.x == 0) {
+        atomicAdd(&h->oversize, 1u);
+        if (!self_sort) h->leaf_sort = q.tot_acc;
+    }
     uint32_t run_acc = 0, run_rej = 0;
     for (uint32_t c0 = q.a0; c0 < q.a1; c0 += kBlock) {
         const uint32_t i = c0 + threadIdx.x;
@@ -951,6 +993,34 @@ __global__ __launch_bounds__(kBlock) void k_leaf_emit_large(TriSoA t, GridHeader
         run_acc += agg_a;
         run_rej += agg_r;
     }
+    if (!self_sort) return;
+    // the accepted pairs of this tile: [pre_acc, pre_acc + run_acc), keys in
+    // [kbase, kbase + span) (leaf_tile_plan guarantees pre_acc + run_acc <= cap)
+    __threadfence_block();
+    __syncthreads();
+    const uint32_t n = run_acc;
+    uint32_t* k0 = keys + q.pre_acc;
+    uint32_t* v0 = vals + q.pre_acc;
+    uint32_t* k1 = tk + q.pre_acc;
+    uint32_t* v1 = tv + q.pre_acc;
+    const uint32_t span = q.span == 0xFFFFFFFFu ? 0xFFFFFFFFu : q.span;
+    const int bits = 32 - __builtin_clz(span | 1u);
+    const int passes = (bits + 7) / 8;
+    const int db = (bits + passes - 1) / passes;
+    for (int p = 0, lo = 0; p < passes; ++p, lo += db) {
+        if (p & 1) block_sort_pass_global(k1, v1, k0, v0, n, q.kbase, lo, min(db, bits - lo), s_base, s_wc, s_scan);
+        else block_sort_pass_global(k0, v0, k1, v1, n, q.kbase, lo, min(db, bits - lo), s_base, s_wc, s_scan);
+        __threadfence_block();
+        __syncthreads();
+    }
+    if (passes & 1) {
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+            k0[i] = k1[i];
+            v0[i] = v1[i];
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += kBlock)
+        write_ref_record(refs, q.pre_acc + i, t.rec, (passes & 1) ? v1[i] : v0[i], L.n_tris);
 }
 
 // Oversize tiles with one thread per top pair: pair i of tile t writes its
@@ -976,7 +1046,10 @@ __global__ __launch_bounds__(kBlock) void k_leaf_emit_par(TriSoA t, GridHeader* 
         const uint32_t tile = reinterpret_cast<const uint2*>(ranges)[key].x / kLeafTile;
         LeafTilePlan q;
         if (tile >= T || !leaf_tile_plan(a, sums, info, T, tile, q) || q.staged) continue;
-        if (i == q.a0) h->leaf_sort = q.tot_acc;
+        if (i == q.a0) {
+            h->leaf_sort = q.tot_acc;
+            atomicAdd(&h->oversize, 1u);
+        }
         uint32_t pa = q.pre_acc + (sacc[i] - sacc[q.a0]);
         uint32_t pr = q.tot_acc + q.pre_rej + (srej[i] - srej[q.a0]);
         const uint32_t tri = top_vals[i];
@@ -1055,8 +1128,8 @@ void launch_leaf_cells(GridHeader* h, const BuildArgs& a, const uint32_t* ranges
 
 void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const uint32_t* top_keys,
                       const uint32_t* top_vals, const uint32_t* ranges, const uint32_t* cells, uint32_t* keys,
-                      uint32_t* vals, float* refs, uint32_t* dbg, uint32_t* sums, uint32_t* info,
-                      const LeafLarge* large, hipStream_t st) {
+                      uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, float* refs, uint32_t* dbg,
+                      uint32_t* sums, uint32_t* info, const LeafLarge* large, hipStream_t st) {
     const uint32_t T = (uint32_t)leaf_tiles(a.cap_P1);
     const uint32_t blocks = (a.cap_P1 + kTile - 1) / kTile;
     if (blocks)
@@ -1074,7 +1147,7 @@ void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const 
                            keys, vals, sums, info, T, large->sacc, large->srej);
     } else {
         hipLaunchKernelGGL(k_leaf_emit_large, dim3(T), dim3(kBlock), 0, st, t, h, a, top_keys, top_vals, cells, keys,
-                           vals, sums, info, T);
+                           vals, sums, info, T, 1, tmp_keys, tmp_vals, refs);
     }
 }
 

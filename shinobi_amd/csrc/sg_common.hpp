@@ -46,7 +46,8 @@ struct alignas(256) GridHeader {
     uint32_t C1, P1, P1_valid, C2, P2, P2_valid;
     uint32_t need_P1, need_C2, need_P2;  // true totals (exceed cap_* on overflow)
     uint32_t status;
-    uint32_t leaf_sort;  // accepted leaf pairs the global sort must order (0: all tiles sorted in LDS)
+    uint32_t leaf_sort;  // accepted leaf pairs the global sort must order (0: no global sort needed)
+    uint32_t oversize;   // leaf tiles too large for the LDS sort met by this build
     // leaf cell size 1/(cell_rcp[d]*c) along d for a top cell of leaf
     // resolution c: the subCellSize of the two-level traversal
     // (src/Control/Algorithm/TwoLevelGridTraverser.h:499-510), computed once

@@ -6,7 +6,8 @@
 namespace sg {
 
 // nine SoA arrays: v0.xyz, v1.xyz, v2.xyz (device pointers), plus the
-// per-triangle 64-byte records (4 x float4) that the bounds pass writes:
+// per-triangle 64-byte records (4 x float4) that k_prep_tris writes whenever
+// the scene's vertices are set (sg_scene_create / sg_scene_update):
 //   {v0.x, v0.y, v0.z, e1.x} {e1.y, e1.z, e2.x, e2.y} {e2.z, v1.x, v1.y, v1.z} {v2.x, v2.y, v2.z, 0}
 // with e1 = v1 - v0, e2 = v2 - v0.  Traversal reads the first 36 bytes, the
 // leaf level of the build gathers v0/v1/v2 from one aligned record instead
@@ -87,8 +88,7 @@ SG_D void write_ref_record(float* __restrict__ refs, uint32_t i, const float4* _
 void launch_ref_records(const uint32_t* indices, const uint32_t* d_valid, const uint32_t* d_gate, uint32_t cap,
                         const float4* rec, uint32_t n_tris, float* refs, hipStream_t st);
 void launch_init(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, hipStream_t st);
-// per-block bounds partials (6 words per block, bounds_blocks(n) blocks) and
-// the triangle records
+// per-block bounds partials (6 words per block, bounds_blocks(n) blocks)
 size_t bounds_blocks(uint32_t n);
 void launch_bounds(const TriSoA& t, uint32_t n, uint32_t* part, hipStream_t st);
 // fold the partials into the header; with derive, resolution + cell sizes
@@ -117,11 +117,15 @@ struct LeafLarge {
     uint32_t *pacc, *prej, *sacc, *srej;
     void* scan_scratch;   // scan_scratch_bytes(cap_P1)
 };
-// refs: reference records of the accepted pairs of LDS-sorted tiles
+// refs: reference records of the accepted pairs of sorted tiles.  Without
+// `large`, a tile too large for LDS is emitted and sorted by one block
+// (using tmp_keys / tmp_vals, buffers of the pairs' size) and no global leaf
+// sort is needed; with `large`, such tiles are emitted in parallel and left
+// to the global sort (launch_leaf_sort_fixup + launch_ref_records).
 void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const uint32_t* top_keys,
                       const uint32_t* top_vals, const uint32_t* ranges, const uint32_t* cells, uint32_t* keys,
-                      uint32_t* vals, float* refs, uint32_t* dbg, uint32_t* sums, uint32_t* info,
-                      const LeafLarge* large, hipStream_t st);
+                      uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, float* refs, uint32_t* dbg,
+                      uint32_t* sums, uint32_t* info, const LeafLarge* large, hipStream_t st);
 // global stable sort of the leaf pairs of tiles too large for LDS (no work
 // when there are none); the result stays in (k0, v0)
 void launch_leaf_sort_fixup(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, GridHeader* h, uint32_t cap,

@@ -17,6 +17,7 @@
 #include <thread>
 #include <vector>
 
+#include "sg_arena.hpp"
 #include "sg_internal.hpp"
 
 namespace {
@@ -314,6 +315,7 @@ struct Request {
     float* t_out = nullptr;
     size_t bytes = 0;            // device bytes while in flight
     int gi = 0;                  // grid of a dynamic scene the request runs on
+    uint64_t version = 0;        // the mesh version of that grid (0 for static scenes)
     GridView view{};             // that grid, read under the server lock at admission
     int trace_block = 64;        // its traversal block size (plain mode)
     Clock::time_point t_submit, t_done;
@@ -343,41 +345,8 @@ struct SceneEntry {
     sg_build_params params{};       // build parameters of a dynamic scene
     hipEvent_t rebuilt = nullptr;
     uint32_t in_use[2] = {0, 0};    // admitted requests that read each grid
-};
-
-// first-fit allocator over [0, size)
-struct Arena {
-    std::map<size_t, size_t> free_;   // offset -> bytes
-    explicit Arena(size_t size = 0) {
-        if (size) free_[0] = size;
-    }
-    bool alloc(size_t bytes, size_t* off) {
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second >= bytes) {
-                *off = it->first;
-                const size_t rest = it->second - bytes, o = it->first + bytes;
-                free_.erase(it);
-                if (rest) free_[o] = rest;
-                return true;
-            }
-        }
-        return false;
-    }
-    void release(size_t off, size_t bytes) {
-        auto it = free_.emplace(off, bytes).first;
-        auto nx = std::next(it);
-        if (nx != free_.end() && it->first + it->second == nx->first) {
-            it->second += nx->second;
-            free_.erase(nx);
-        }
-        if (it != free_.begin()) {
-            auto pv = std::prev(it);
-            if (pv->first + pv->second == it->first) {
-                pv->second += it->second;
-                free_.erase(it);
-            }
-        }
-    }
+    uint64_t version[2] = {0, 0};   // mesh version each grid holds: k after the k-th update
+    uint64_t updates = 0;           // successful update_dynamic calls so far
 };
 
 size_t request_bytes(uint32_t n) { return align_up((size_t)n * 4, kAlign) * 9; }
@@ -571,6 +540,7 @@ void serve_loop(sg_server* s) {
                 for (Request* r : sl.reqs) {
                     SceneEntry& se = s->scenes[r->scene];
                     r->gi = se.dynamic ? se.front : 0;
+                    r->version = se.dynamic ? se.version[r->gi] : 0;
                     se.in_use[r->gi]++;
                     r->view = grid_view(se.grid[r->gi]);
                     r->trace_block = se.grid[r->gi]->p.trace_block;
@@ -763,6 +733,7 @@ sg_status sg_server_update_dynamic(sg_server* s, uint32_t id, const float* const
         return r;
     }
     se.updating = false;
+    se.version[back] = ++se.updates;
     se.swap_pending = true;
     s->cv_work.notify_all();
     // returns once requests admitted from now on see the new mesh
@@ -805,6 +776,10 @@ sg_status sg_server_submit(sg_server* s, uint32_t scene, const float* const rays
 }
 
 sg_status sg_server_wait(sg_server* s, uint64_t id, double* latency_us) {
+    return sg_server_wait_version(s, id, latency_us, nullptr);
+}
+
+sg_status sg_server_wait_version(sg_server* s, uint64_t id, double* latency_us, uint64_t* version) {
     if (!s) return SG_ERR_INVALID_ARG;
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->live.find(id);
@@ -812,6 +787,7 @@ sg_status sg_server_wait(sg_server* s, uint64_t id, double* latency_us) {
     Request* r = it->second;
     s->cv_done.wait(lk, [&] { return r->done; });
     if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(r->t_done - r->t_submit).count();
+    if (version) *version = r->version;
     const sg_status st = r->status;
     s->live.erase(it);
     delete r;

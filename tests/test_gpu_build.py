@@ -119,6 +119,14 @@ def test_rebuild_into_an_oversize_tile(ctx, oracle):
     assert tuple(info.res) == tuple(og.res)
     _assert_top_level(g, og)
     _assert_leaf_level(g, og)
+    # the traversal reads the reference records the block wrote for the
+    # oversize tile: rays through the dense cell hit what the oracle hits
+    rays = sa.gen_random_rays(72, 30_000, 0.28, 0.32)
+    pg, tg = g.trace_host(rays)
+    po, to = og.trace(rays)
+    assert (po != sa.MISS).sum() > 1000
+    np.testing.assert_array_equal(pg, po)
+    np.testing.assert_array_equal(tg.view(np.uint32), to.view(np.uint32))
 
 
 def test_twolevel_reference_leaf_lambda(ctx, oracle):
