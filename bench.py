@@ -13,7 +13,8 @@ Workload (one step = one pass of the hot path over one batch):
             in 2x32 pixel tiles; hits and rays are written at their pixel
             ids (the row-major visit is reported as trace_primary_rowmajor_ms)
          3. one cosine-weighted diffuse bounce per hit (device generator,
-            compacted and ordered by direction class in rounds of 256)
+            compacted and ordered longest first: descending exit distance
+            from the grid's bounds, sg_bounce_ordered)
          4. closest-hit trace of the bounce rays (count read on the device)
 value  = closest-hit Mrays/s = (primary + bounce rays) / (time of the two
          trace kernels), measured with HIP events on the stream the kernels
@@ -157,6 +158,16 @@ def _log_uniform(rng, lo, hi, n):
     return np.exp(rng.uniform(np.log(lo), np.log(hi), n)).astype(np.int64)
 
 
+def wait_until(t: float) -> None:
+    """sleep / yield until perf_counter() reaches t without holding the GIL
+    in a busy loop (the dynamic-mesh thread must run on schedule)"""
+    while True:
+        left = t - time.perf_counter()
+        if left <= 0:
+            return
+        time.sleep(left - 1e-4 if left > 3e-4 else 0)
+
+
 def serving_bench(sa, W, ctx, seconds: float = 2.0, rate: float = 2000.0, burst: int = 400) -> dict:
     """Served ray queries: 16 resident static scenes (50k-2M triangles,
     uniform soups and height fields) plus a 200k-triangle sphere rebuilt every
@@ -203,6 +214,7 @@ def serving_bench(sa, W, ctx, seconds: float = 2.0, rate: float = 2000.0, burst:
         ids = [srv.add_grid(g) for _, g in grids]
         ids.append(srv.add_dynamic(frames[0], lam=5.0, lambda_leaf=2.0))
         stop = threading.Event()
+        updates = []
 
         def producer():
             f = 0
@@ -210,6 +222,7 @@ def serving_bench(sa, W, ctx, seconds: float = 2.0, rate: float = 2000.0, burst:
                 t0 = time.perf_counter()
                 f += 1
                 srv.update_dynamic(ids[16], frames[f % len(frames)])
+                updates.append(time.perf_counter())
                 stop.wait(max(0.0, 0.033 - (time.perf_counter() - t0)))
 
         th = threading.Thread(target=producer, daemon=True)
@@ -217,21 +230,26 @@ def serving_bench(sa, W, ctx, seconds: float = 2.0, rate: float = 2000.0, burst:
         srv.reset_stats()
         t = time.perf_counter()
         n_sub = n_req if timed else burst
+        t_begin = time.perf_counter()
         for k in range(n_sub):
             if timed:
                 t += gaps[k]
-                while time.perf_counter() < t:
-                    pass
+                wait_until(t)
             r, outs = temps[which[k]]
             o = outs[k & 1]
             srv.submit(ids[scene_of[k]], r.a, o[0].a, o[1].a)
         srv.drain()
+        t_end = time.perf_counter()
         st = srv.stats()
         stop.set()
         th.join()
         srv.close()
+        during = [u for u in updates if t_begin <= u <= t_end]
+        gaps_ms = np.diff(during) * 1e3 if len(during) > 1 else np.array([0.0])
         return {"requests": st.requests, "p50_us": round(st.p50_us, 1), "p99_us": round(st.p99_us, 1),
-                "mean_us": round(st.mean_us, 1), "mrays_per_s": round(st.mrays_per_s, 1), "waves": st.waves}
+                "mean_us": round(st.mean_us, 1), "mrays_per_s": round(st.mrays_per_s, 1), "waves": st.waves,
+                "mesh_updates": len(during), "update_interval_ms_mean": round(float(np.mean(gaps_ms)), 1),
+                "update_interval_ms_max": round(float(np.max(gaps_ms)), 1)}
 
     out = {"scenes": {"static_tris": n_tris, "dynamic_tris": int(frames[0].shape[1]), "rebuild_ms_period": 33},
            "offered_req_per_s": rate,
@@ -250,7 +268,7 @@ def serving_bench(sa, W, ctx, seconds: float = 2.0, rate: float = 2000.0, burst:
 def cluster_bench(sa, W, ctx, spp: int = 4, depth: int = 4, reps: int = 3) -> dict:
     """Dense cluster in a large scene (500k tiny + 2k large triangles):
     two-level rebuild, then 1280x720 x `spp` camera paths of `depth`
-    diffuse segments chained on the device."""
+    diffuse segments chained on the device (bounces longest first)."""
     w = W.cluster()
     scene = sa.FaceSoup(ctx, w.scene)
     b = sa.FastTwoLevelGridBuilder(ctx, **w.build)
@@ -258,6 +276,7 @@ def cluster_bench(sa, W, ctx, spp: int = 4, depth: int = 4, reps: int = 3) -> di
     g = b.build()
     info = g.info()
     rays = sa.RayBatch(ctx, w.rays)
+    order = sa.bounce_order("exit", grid=g)
     e0, e1, e2 = ctx.event(), ctx.event(), ctx.event()
     best = None
     for rep in range(reps + 1):
@@ -266,7 +285,7 @@ def cluster_bench(sa, W, ctx, spp: int = 4, depth: int = 4, reps: int = 3) -> di
         e1.record()
         counts = []
         for smp in range(spp):
-            counts += sa.trace_paths(ctx, scene, g, rays, depth, w.bounce_seed + 16 * smp)
+            counts += sa.trace_paths(ctx, scene, g, rays, depth, w.bounce_seed + 16 * smp, order=order)
         e2.record()
         ctx.sync()
         n_rays = spp * rays.n + sum(int(c.download(np.uint32, 1)[0]) for c in counts)
@@ -297,6 +316,7 @@ def ruins_bench(sa, W, ctx, reps: int = 10) -> dict:
     out = sa.RayBatch(ctx, capacity=n)
     bp, bt = sa.DeviceBuffer(ctx, 4 * n), sa.DeviceBuffer(ctx, 4 * n)
     count = sa.DeviceBuffer(ctx, 4)
+    order = sa.bounce_order("exit", grid=g)
     ev = [ctx.event() for _ in range(5)]
     rows = []
     for rep in range(reps + 2):
@@ -305,7 +325,7 @@ def ruins_bench(sa, W, ctx, reps: int = 10) -> dict:
         ev[1].record()
         g.trace_camera(cam, PRIMARY_TILE, prim=prim, t=t, rays_out=rays)
         ev[2].record()
-        sa.bounce(ctx, scene, rays, prim, t, w.bounce_seed, out, count=count)
+        sa.bounce(ctx, scene, rays, prim, t, w.bounce_seed, out, count=count, order=order)
         ev[3].record()
         g.trace(out, prim=bp, t=bt, d_count=count.ptr, n=n)
         ev[4].record()
@@ -462,7 +482,8 @@ def main() -> int:
     if not np.array_equal(rays.download().view(np.uint32), w.rays.view(np.uint32)):
         raise RuntimeError("device camera rays differ from the host generator")
     grid.trace(rays, stats=st1)
-    sa.bounce(ctx, scene, rays, prim, tbuf, w.bounce_seed, brays, count=count)
+    order = sa.bounce_order("exit", grid=grid)
+    sa.bounce(ctx, scene, rays, prim, tbuf, w.bounce_seed, brays, count=count, order=order)
     grid.trace(brays, prim=bprim, t=bt, stats=st2, d_count=count.ptr, n=n_rays)
     m = int(count.download(np.uint32, 1)[0])
     f_primary = trace_flops(st1.download(np.uint32, 4 * n_rays).reshape(-1, 4))
@@ -477,7 +498,7 @@ def main() -> int:
         grid.rebuild_phased(e[:9])
         grid.trace_camera(cam, PRIMARY_TILE, prim=prim, t=tbuf, rays_out=rays)
         e[9].record()
-        sa.bounce(ctx, scene, rays, prim, tbuf, w.bounce_seed, brays, count=count)
+        sa.bounce(ctx, scene, rays, prim, tbuf, w.bounce_seed, brays, count=count, order=order)
         e[10].record()
         grid.trace(brays, prim=bprim, t=bt, d_count=count.ptr, n=n_rays)
         e[11].record()

@@ -259,6 +259,33 @@ sg_status sg_trace_camera(sg_ctx *ctx, sg_grid *grid, const sg_pinhole *cam, uin
 sg_status sg_bounce(sg_ctx *ctx, sg_scene *scene, const sg_rays *rays, const uint32_t *d_prim,
                     const float *d_t, uint32_t n, const uint32_t *d_n, uint64_t seed, const sg_rays_out *out,
                     uint32_t *d_src, uint32_t *d_count, void *stream);
+/* Orders of the bounce rays of sg_bounce_ordered.  Ray order changes no
+ * result (each ray's hit is its own); it decides which rays share a wave and
+ * when the long ones start, and a launch lasts about as long as its longest
+ * rays.  No counterpart in the reference, whose path tracer keeps pixel order.
+ *   SG_BOUNCE_CLASS_ROUNDS  sg_bounce's order (direction classes in rounds)
+ *   SG_BOUNCE_EXIT_FIRST    longest first: descending exit distance e of
+ *                           the ray from the box [box_min, box_max] (pass the
+ *                           grid's bounds; per axis the far slab distance
+ *                           fmaxf((min - o) / d, (max - o) / d), the smallest
+ *                           of the three) in 65536 steps of the box diagonal:
+ *                           key 65535 - min(65535, (uint)(max(e, 0) * 65535 /
+ *                           diagonal)), diagonal = sqrtf((dx*dx + dy*dy) + dz*dz)
+ *   SG_BOUNCE_AXIS_FIRST    ascending |d . axis| in 65536 steps: key
+ *                           min(65535, (uint)(|d . axis| * 65535)); rays
+ *                           running across `axis` first (height fields: up)
+ *   Keys ascending, ties in input order; all arithmetic binary32. */
+enum { SG_BOUNCE_CLASS_ROUNDS = 0, SG_BOUNCE_EXIT_FIRST = 1, SG_BOUNCE_AXIS_FIRST = 2 };
+typedef struct {
+    int order;
+    float box_min[3], box_max[3];
+    float axis[3];
+} sg_bounce_order;
+/* sg_bounce with the rays in the given order (same rays, same src mapping) */
+sg_status sg_bounce_ordered(sg_ctx *ctx, sg_scene *scene, const sg_rays *rays, const uint32_t *d_prim,
+                            const float *d_t, uint32_t n, const uint32_t *d_n, uint64_t seed,
+                            const sg_bounce_order *order, const sg_rays_out *out, uint32_t *d_src,
+                            uint32_t *d_count, void *stream);
 /* sg_bounce that allocates nothing: no counterpart in the reference.  For
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and

@@ -95,6 +95,10 @@ def load() -> C.CDLL:
     lib.or_brute_f64.argtypes = [P9, C.c_uint32, P7, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int]
     lib.or_bounce.restype = C.c_uint32
     lib.or_bounce.argtypes = [P9, P7, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64, P7, C.c_void_p]
+    lib.or_bounce_ordered.restype = C.c_uint32
+    F3 = C.c_float * 3
+    lib.or_bounce_ordered.argtypes = [P9, P7, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, F3, F3, F3,
+                                      P7, C.c_void_p]
     lib.or_hash3.restype = C.c_uint64
     lib.or_hash3.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
     lib.or_u01.restype = C.c_float
@@ -230,7 +234,10 @@ def brute_f64(soa9, rays7, nthreads=8):
     return prim, t
 
 
-def bounce(soa9, rays7, prim, t, seed):
+def bounce(soa9, rays7, prim, t, seed, order=None, box=None, axis=None):
+    """bounce rays of every hit; order None: direction-class rounds, "exit":
+    longest exit distance from box = (min, max) first, "axis": smallest
+    |d . axis| first"""
     soa9 = np.ascontiguousarray(soa9, np.float32)
     rays7 = np.ascontiguousarray(rays7, np.float32)
     prim = np.ascontiguousarray(prim, np.uint32)
@@ -238,8 +245,16 @@ def bounce(soa9, rays7, prim, t, seed):
     n = rays7.shape[1]
     out = np.empty((7, max(n, 1)), np.float32)
     src = np.empty(max(n, 1), np.uint32)
-    m = load().or_bounce(_p9(soa9), _p7(rays7), prim.ctypes.data, t.ctypes.data, n, seed, _p7(out),
-                         src.ctypes.data)
+    if order is None:
+        m = load().or_bounce(_p9(soa9), _p7(rays7), prim.ctypes.data, t.ctypes.data, n, seed, _p7(out),
+                             src.ctypes.data)
+    else:
+        F3 = C.c_float * 3
+        lo, hi = box if box is not None else ((0.0,) * 3, (0.0,) * 3)
+        ax = axis if axis is not None else (0.0, 0.0, 0.0)
+        m = load().or_bounce_ordered(_p9(soa9), _p7(rays7), prim.ctypes.data, t.ctypes.data, n, seed,
+                                     {"exit": 1, "axis": 2}[order], F3(*lo), F3(*hi), F3(*ax), _p7(out),
+                                     src.ctypes.data)
     return out[:, :m].copy(), src[:m].copy()
 
 

@@ -1111,14 +1111,11 @@ static uint32_t dir_bin(float c)
     return f < 1.f ? 0u : (f < 2.f ? 1u : (f < 3.f ? 2u : 3u));
 }
 
-uint32_t or_bounce(const float *const v[9], const float *const rays[7], const uint32_t *prim,
-                   const float *t, uint32_t nrays, uint64_t seed, float *const out[7], uint32_t *src)
+/* the bounce rays of every hit in input order: 7 floats per ray into tmp,
+ * the input ray index into tsrc; returns their number */
+static uint32_t gen_bounces(const float *const v[9], const float *const rays[7], const uint32_t *prim,
+                            const float *t, uint32_t nrays, uint64_t seed, float *tmp, uint32_t *tsrc)
 {
-    /* the bounce rays in input order, then ordered by (r / 256, class, r % 256),
-     * r the ray's rank within its direction class in input order */
-    float *tmp = (float *)malloc((size_t)7 * (nrays ? nrays : 1) * sizeof(float));
-    uint32_t *tsrc = (uint32_t *)malloc((size_t)(nrays ? nrays : 1) * sizeof(uint32_t));
-    uint32_t *cls = (uint32_t *)malloc((size_t)(nrays ? nrays : 1) * sizeof(uint32_t));
     uint32_t m = 0;
     for (uint32_t i = 0; i < nrays; ++i) {
         if (prim[i] == OR_MISS) continue;
@@ -1143,8 +1140,23 @@ uint32_t or_bounce(const float *const v[9], const float *const rays[7], const ui
         r[3] = nd.x; r[4] = nd.y; r[5] = nd.z;
         r[6] = FLT_MAX;
         tsrc[m] = i;
-        cls[m] = dir_bin(nd.x) | dir_bin(nd.y) << 2 | dir_bin(nd.z) << 4;
         ++m;
+    }
+    return m;
+}
+
+uint32_t or_bounce(const float *const v[9], const float *const rays[7], const uint32_t *prim,
+                   const float *t, uint32_t nrays, uint64_t seed, float *const out[7], uint32_t *src)
+{
+    /* the bounce rays in input order, then ordered by (r / 256, class, r % 256),
+     * r the ray's rank within its direction class in input order */
+    float *tmp = (float *)malloc((size_t)7 * (nrays ? nrays : 1) * sizeof(float));
+    uint32_t *tsrc = (uint32_t *)malloc((size_t)(nrays ? nrays : 1) * sizeof(uint32_t));
+    uint32_t *cls = (uint32_t *)malloc((size_t)(nrays ? nrays : 1) * sizeof(uint32_t));
+    const uint32_t m = gen_bounces(v, rays, prim, t, nrays, seed, tmp, tsrc);
+    for (uint32_t k = 0; k < m; ++k) {
+        const float *r = tmp + (size_t)7 * k;
+        cls[k] = dir_bin(r[3]) | dir_bin(r[4]) << 2 | dir_bin(r[5]) << 4;
     }
     uint32_t tot[64] = {0}, seen[64] = {0};
     for (uint32_t k = 0; k < m; ++k) ++tot[cls[k]];
@@ -1165,5 +1177,51 @@ uint32_t or_bounce(const float *const v[9], const float *const rays[7], const ui
     free(tmp);
     free(tsrc);
     free(cls);
+    return m;
+}
+
+/* f -> [0, 65535]: 0 for NaN and f <= 0, else min(65535, (uint32)f) */
+static uint32_t quant16(float f)
+{
+    if (!(f > 0.f)) return 0u;
+    return f >= 65535.f ? 65535u : (uint32_t)f;
+}
+
+uint32_t or_bounce_ordered(const float *const v[9], const float *const rays[7], const uint32_t *prim,
+                           const float *t, uint32_t nrays, uint64_t seed, int order, const float box_min[3],
+                           const float box_max[3], const float axis[3], float *const out[7], uint32_t *src)
+{
+    float *tmp = (float *)malloc((size_t)7 * (nrays ? nrays : 1) * sizeof(float));
+    uint32_t *tsrc = (uint32_t *)malloc((size_t)(nrays ? nrays : 1) * sizeof(uint32_t));
+    uint32_t *key = (uint32_t *)malloc((size_t)(nrays ? nrays : 1) * sizeof(uint32_t));
+    uint32_t *idx = (uint32_t *)malloc((size_t)(nrays ? nrays : 1) * sizeof(uint32_t));
+    const uint32_t m = gen_bounces(v, rays, prim, t, nrays, seed, tmp, tsrc);
+    const float ddx = box_max[0] - box_min[0], ddy = box_max[1] - box_min[1], ddz = box_max[2] - box_min[2];
+    const float scale = 65535.f / sqrtf((ddx * ddx + ddy * ddy) + ddz * ddz);
+    for (uint32_t k = 0; k < m; ++k) {
+        const float *r = tmp + (size_t)7 * k;
+        const v3 o = mk(r[0], r[1], r[2]), d = mk(r[3], r[4], r[5]);
+        if (order == 2) {
+            key[k] = quant16(fabsf(dot(d, mk(axis[0], axis[1], axis[2]))) * 65535.f);
+        } else {
+            /* far slab distance per axis, the smallest of the three: the
+             * distance at which the ray leaves the box, in steps of the
+             * diagonal / 65535, longest first */
+            const float fx = fmaxf((box_min[0] - o.x) / d.x, (box_max[0] - o.x) / d.x);
+            const float fy = fmaxf((box_min[1] - o.y) / d.y, (box_max[1] - o.y) / d.y);
+            const float fz = fmaxf((box_min[2] - o.z) / d.z, (box_max[2] - o.z) / d.z);
+            key[k] = 65535u - quant16(fmaxf(fminf(fminf(fx, fy), fz), 0.f) * scale);
+        }
+        idx[k] = k;
+    }
+    or_sort_pairs(key, idx, m, 4);   /* stable: ties keep input order */
+    for (uint32_t j = 0; j < m; ++j) {
+        for (int a = 0; a < 7; ++a) out[a][j] = tmp[(size_t)7 * idx[j] + a];
+        src[j] = tsrc[idx[j]];
+    }
+    free(tmp);
+    free(tsrc);
+    free(key);
+    free(idx);
     return m;
 }

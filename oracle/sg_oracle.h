@@ -144,6 +144,16 @@ uint32_t or_bounce(const float *const v[9], const float *const rays[7],
                    const uint32_t *prim, const float *t, uint32_t nrays,
                    uint64_t seed, float *const out[7], uint32_t *src);
 
+/* The same bounce rays ordered by a 16-bit key, smallest first, ties in
+ * input order (the library's sg_bounce_ordered): order 1 = 65535 -
+ * min(65535, (uint)(max(e, 0) * 65535 / diagonal)), e the exit distance from
+ * the box [box_min, box_max] (per axis fmaxf((min - o) / d, (max - o) / d),
+ * the smallest of the three), diagonal = sqrtf((dx*dx + dy*dy) + dz*dz);
+ * order 2 = min(65535, (uint)(|d . axis| * 65535)). */
+uint32_t or_bounce_ordered(const float *const v[9], const float *const rays[7], const uint32_t *prim,
+                           const float *t, uint32_t nrays, uint64_t seed, int order, const float box_min[3],
+                           const float box_max[3], const float axis[3], float *const out[7], uint32_t *src);
+
 uint64_t or_hash3(uint64_t seed, uint64_t a, uint64_t b);
 float    or_u01(uint64_t h);
 

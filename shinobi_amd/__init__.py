@@ -197,6 +197,9 @@ class PinnedArray:
         check(ctx._l.sg_host_alloc(ctx.h, max(n, 1), C.byref(self.ptr)), "sg_host_alloc")
         ctx._own(self)
         buf = (C.c_char * max(n, 1)).from_address(self.ptr.value)
+        # the array keeps this object (and so the pinned memory) alive: a
+        # view of .a outliving its PinnedArray must not read freed memory
+        buf._owner = self
         self.a = np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
 
     def close(self) -> None:
@@ -474,28 +477,52 @@ class FastTwoLevelGridBuilder(_Builder):
     KIND = L.SG_GRID_TWO_LEVEL
 
 
+def bounce_order(order: str = "class", grid: "Grid | None" = None, box=None, axis=None) -> L.BounceOrder:
+    """sg_bounce_order: "class" (sg_bounce's direction-class rounds), "exit"
+    (longest exit distance from the grid's bounds, or `box` = (min, max),
+    first) or "axis" (smallest |d . axis| first)."""
+    q = L.BounceOrder()
+    q.order = {"class": L.SG_BOUNCE_CLASS_ROUNDS, "exit": L.SG_BOUNCE_EXIT_FIRST, "axis": L.SG_BOUNCE_AXIS_FIRST}[order]
+    if order == "exit":
+        if box is None:
+            i = grid.info()
+            box = (tuple(i.bmin), tuple(i.bmax))
+        for k in range(3):
+            q.box_min[k], q.box_max[k] = box[0][k], box[1][k]
+    if order == "axis":
+        for k in range(3):
+            q.axis[k] = axis[k]
+    return q
+
+
 def bounce(ctx: Context, scene: FaceSoup, rays: RayBatch, prim: DeviceBuffer, t: DeviceBuffer, seed: int,
            out: RayBatch, src: DeviceBuffer | None = None, count: DeviceBuffer | None = None,
-           d_n: C.c_void_p | None = None, stream=None) -> DeviceBuffer:
+           d_n: C.c_void_p | None = None, stream=None, order: L.BounceOrder | None = None) -> DeviceBuffer:
     """One cosine-weighted diffuse bounce per hit, compacted; returns the
     device counter holding the number of bounce rays.  d_n: optional device
-    count of the input rays (<= rays.n); stream: defaults to the context's."""
+    count of the input rays (<= rays.n); stream: defaults to the context's;
+    order: a bounce_order() (default: sg_bounce's direction-class rounds)."""
     count = count or DeviceBuffer(ctx, 4)
     r = rays.struct()
     o = out.struct()
-    check(ctx._l.sg_bounce(ctx.h, scene.h, C.byref(r), prim.ptr, t.ptr, rays.n, d_n, seed, C.byref(o),
-                           src.ptr if src is not None else None, count.ptr,
-                           ctx.stream if stream is None else stream), "sg_bounce")
+    st = ctx.stream if stream is None else stream
+    sp = src.ptr if src is not None else None
+    if order is None:
+        check(ctx._l.sg_bounce(ctx.h, scene.h, C.byref(r), prim.ptr, t.ptr, rays.n, d_n, seed, C.byref(o), sp,
+                               count.ptr, st), "sg_bounce")
+    else:
+        check(ctx._l.sg_bounce_ordered(ctx.h, scene.h, C.byref(r), prim.ptr, t.ptr, rays.n, d_n, seed,
+                                       C.byref(order), C.byref(o), sp, count.ptr, st), "sg_bounce_ordered")
     return count
 
 
 def trace_paths(ctx: Context, scene: FaceSoup, grid: "Grid", rays: RayBatch, depth: int, seed: int,
-                keep: bool = False):
+                keep: bool = False, order: L.BounceOrder | None = None):
     """Diffuse paths: trace the rays, bounce every hit, trace the bounce rays,
     ... `depth` traced segments in all, every count kept on the device
-    (sg_bounce with a device input count).  Segment d > 0 uses bounce seed
-    seed + d - 1.  Returns the device counters of segments 1..depth-1 (and,
-    with keep, each segment's (rays, prim, t))."""
+    (sg_bounce with a device input count, in `order` when given).  Segment
+    d > 0 uses bounce seed seed + d - 1.  Returns the device counters of
+    segments 1..depth-1 (and, with keep, each segment's (rays, prim, t))."""
     n = rays.n
     bufs = [RayBatch(ctx, capacity=n), RayBatch(ctx, capacity=n)]
     prims = [DeviceBuffer(ctx, 4 * n) for _ in range(2)]
@@ -510,7 +537,7 @@ def trace_paths(ctx: Context, scene: FaceSoup, grid: "Grid", rays: RayBatch, dep
             segs.append((cur, p, t, cur_n))
         if d + 1 < depth:
             nxt = bufs[k] if not keep else RayBatch(ctx, capacity=n)
-            bounce(ctx, scene, cur, p, t, seed + d, nxt, count=counts[d], d_n=cur_n)
+            bounce(ctx, scene, cur, p, t, seed + d, nxt, count=counts[d], d_n=cur_n, order=order)
             cur, cur_n, k = nxt, counts[d].ptr, k ^ 1
     return (counts, segs) if keep else counts
 

@@ -109,6 +109,14 @@ class GridDeviceView(C.Structure):
                 ("indices", C.c_void_p), ("n_refs", C.c_uint32), ("tri", C.c_void_p * 9), ("n_tris", C.c_uint32)]
 
 
+SG_BOUNCE_CLASS_ROUNDS, SG_BOUNCE_EXIT_FIRST, SG_BOUNCE_AXIS_FIRST = 0, 1, 2
+
+
+class BounceOrder(C.Structure):
+    _fields_ = [("order", C.c_int), ("box_min", C.c_float * 3), ("box_max", C.c_float * 3),
+                ("axis", C.c_float * 3)]
+
+
 class ServerStats(C.Structure):
     _fields_ = [("requests", C.c_uint64), ("rays", C.c_uint64), ("waves", C.c_uint64),
                 ("p50_us", C.c_double), ("p99_us", C.c_double), ("mean_us", C.c_double), ("max_us", C.c_double),
@@ -147,6 +155,9 @@ SIGNATURES = {
     "sg_pinhole_init": (C.c_int, [Float3, Float3, Float3, C.c_float, C.c_uint32, C.c_uint32, C.POINTER(Pinhole)]),
     "sg_trace_camera": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Pinhole), C.c_uint32, C.c_uint32, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_void_p]),
+    "sg_bounce_ordered": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_uint32,
+                                    C.c_void_p, C.c_uint64, C.POINTER(BounceOrder), C.POINTER(Rays), C.c_void_p,
+                                    C.c_void_p, C.c_void_p]),
     "sg_bounce_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_uint32,
                                     C.c_void_p, C.c_uint64, C.POINTER(Rays), C.c_void_p, C.c_void_p, C.c_void_p]),
     "sg_scan_u32": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]),

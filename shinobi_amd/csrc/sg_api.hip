@@ -732,6 +732,41 @@ sg_status sg_bounce(sg_ctx* ctx, sg_scene* scene, const sg_rays* rays, const uin
     return SG_OK;
 }
 
+sg_status sg_bounce_ordered(sg_ctx* ctx, sg_scene* scene, const sg_rays* rays, const uint32_t* d_prim,
+                            const float* d_t, uint32_t n, const uint32_t* d_n, uint64_t seed,
+                            const sg_bounce_order* order, const sg_rays_out* out, uint32_t* d_src, uint32_t* d_count,
+                            void* stream) {
+    if (!order) return SG_ERR_INVALID_ARG;
+    if (order->order == SG_BOUNCE_CLASS_ROUNDS)
+        return sg_bounce(ctx, scene, rays, d_prim, d_t, n, d_n, seed, out, d_src, d_count, stream);
+    if (order->order != SG_BOUNCE_EXIT_FIRST && order->order != SG_BOUNCE_AXIS_FIRST) return SG_ERR_INVALID_ARG;
+    if (!ctx || !scene || !rays || !d_prim || !d_t || !out || !d_count) return SG_ERR_INVALID_ARG;
+    if (n >= (1u << 30)) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    clear_last_error();
+    void* buf = nullptr;
+    sg_status r = ctx_scratch(ctx, (hipStream_t)stream, bounce_ordered_scratch_bytes(n), &buf);
+    if (r != SG_OK) return r;
+    BounceOrder q{};
+    q.mode = order->order == SG_BOUNCE_EXIT_FIRST ? BOUNCE_EXIT_FIRST : BOUNCE_AXIS_FIRST;
+    for (int k = 0; k < 3; ++k) {
+        q.bmin[k] = order->box_min[k];
+        q.bmax[k] = order->box_max[k];
+        q.axis[k] = order->axis[k];
+    }
+    q.scale = bounce_exit_scale(q.bmin, q.bmax);
+    RaySoAOut o;
+    for (int i = 0; i < 3; ++i) {
+        o.a[i] = out->o[i];
+        o.a[3 + i] = out->d[i];
+    }
+    o.a[6] = out->tmax;
+    launch_bounce_ordered(scene_soa(scene), ray_soa(rays), d_prim, d_t, n, d_n, seed, q, o, d_src, d_count, buf,
+                          (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return SG_OK;
+}
+
 // sg_bounce with the library's fixed per-device scratch array: no allocation
 // of any kind, so it may be called under graph capture at any size up to
 // the limit

@@ -216,6 +216,24 @@ void launch_trace_gen(const GridView& g, const RaySoA& r, const CamArgs* cam, ui
 // class in rounds of 256 rays (input order within a class); *d_count
 // receives the number of bounce rays.  scratch: bounce_scratch_bytes(n).
 size_t bounce_scratch_bytes(uint32_t n);
+// Bounce rays sorted by a key instead (see sg_bounce_ordered): compaction in
+// input order, a stable sort of the keys, a gather.
+enum { BOUNCE_CLASS_ROUNDS = 0, BOUNCE_EXIT_FIRST = 1, BOUNCE_AXIS_FIRST = 2 };
+struct BounceOrder {
+    int mode;
+    float bmin[3], bmax[3];   // BOUNCE_EXIT_FIRST: the box the exit distance is measured in
+    float scale;              // 65535 / the box diagonal (bounce_exit_scale)
+    float axis[3];            // BOUNCE_AXIS_FIRST
+};
+// 65535 / sqrtf((dx * dx + dy * dy) + dz * dz), d = bmax - bmin, binary32
+inline float bounce_exit_scale(const float bmin[3], const float bmax[3]) {
+    const float dx = bmax[0] - bmin[0], dy = bmax[1] - bmin[1], dz = bmax[2] - bmin[2];
+    return 65535.f / sqrtf((dx * dx + dy * dy) + dz * dz);
+}
+size_t bounce_ordered_scratch_bytes(uint32_t n);
+void launch_bounce_ordered(const TriSoA& tris, const RaySoA& r, const uint32_t* prim, const float* t, uint32_t n,
+                           const uint32_t* d_n, uint64_t seed, const BounceOrder& q, const RaySoAOut& out,
+                           uint32_t* src, uint32_t* d_count, void* scratch, hipStream_t st);
 void launch_bounce(const TriSoA& tris, const RaySoA& r, const uint32_t* prim, const float* t, uint32_t n,
                    const uint32_t* d_n, uint64_t seed, const RaySoAOut& out, uint32_t* src, uint32_t* d_count,
                    void* scratch, hipStream_t st);

@@ -1020,6 +1020,130 @@ __global__ __launch_bounds__(kBlock) void k_bounce(TriSoA tris, RaySoA rays, con
     if (src) src[m] = i;
 }
 
+__global__ void k_set_u32(uint32_t* p, uint32_t v);
+
+// ------------------------------------------------------ ordered bounce rays
+// Bounce rays in an order given by a per-ray 16-bit key: compaction in input
+// order (k_bounce_hits / k_bounce_scan), the rays and their keys written at
+// their compacted slots (k_bounce_keyed), a stable two-pass radix sort of
+// (key, slot) pairs, then a gather into the output (k_bounce_gather).
+// Keys (smaller first, ties in input order):
+//   BOUNCE_EXIT_FIRST: 65535 - q, q = min(65535, (uint)(max(e, 0) * scale)),
+//     e the exit distance of the ray from the box [bmin, bmax] (per axis the
+//     far slab distance fmaxf((bmin - o) / d, (bmax - o) / d), the smallest
+//     over x, y, z) and scale = 65535 / the box diagonal: longest first.
+//   BOUNCE_AXIS_FIRST: min(65535, (uint)(|d . axis| * 65535)).
+// A launch lasts about as long as its longest rays; starting them first (and
+// keeping rays of similar length, hence similar paths, in one wave) shortens
+// it.  Sixteen bits order the rays as well as the full distance does
+// (measured on three scenes) in half the sort passes.
+SG_D uint32_t quant16(float f) {   // NaN and f <= 0 -> 0
+    if (!(f > 0.f)) return 0u;
+    return f >= 65535.f ? 65535u : (uint32_t)f;
+}
+
+SG_D uint32_t bounce_key(const BounceOrder& q, v3 o, v3 d) {
+    if (q.mode == BOUNCE_AXIS_FIRST)
+        return quant16(fabsf(dot(d, mk(q.axis[0], q.axis[1], q.axis[2]))) * 65535.f);
+    const float fx = fmaxf((q.bmin[0] - o.x) / d.x, (q.bmax[0] - o.x) / d.x);
+    const float fy = fmaxf((q.bmin[1] - o.y) / d.y, (q.bmax[1] - o.y) / d.y);
+    const float fz = fmaxf((q.bmin[2] - o.z) / d.z, (q.bmax[2] - o.z) / d.z);
+    return 65535u - quant16(fmaxf(fminf(fminf(fx, fy), fz), 0.f) * q.scale);
+}
+
+// hits per tile: counts[tile] (one class)
+__global__ __launch_bounds__(kBlock) void k_bounce_hits(const uint32_t* __restrict__ prim, uint32_t cap,
+                                                        const uint32_t* d_n, uint32_t* __restrict__ counts) {
+    __shared__ uint32_t s_scan[kBlock / 64];
+    const uint32_t n = d_n ? min(*d_n, cap) : cap;
+    const uint32_t i = blockIdx.x * kBounceTile + threadIdx.x;
+    const uint32_t hit = (i < n && prim[i] != kMiss) ? 1u : 0u;
+    const uint32_t c = block_sum<kBlock / 64>(hit, s_scan);
+    if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bounce_keyed(TriSoA tris, RaySoA rays, const uint32_t* __restrict__ prim,
+                                                         const float* __restrict__ tin, uint32_t cap,
+                                                         const uint32_t* d_n, uint64_t seed, BounceOrder q,
+                                                         RaySoAOut tmp, uint32_t* __restrict__ keys,
+                                                         uint32_t* __restrict__ vals, uint32_t* __restrict__ tsrc,
+                                                         const uint32_t* __restrict__ counts,
+                                                         const uint32_t* __restrict__ totals, uint32_t* d_count) {
+    __shared__ uint32_t s_scan[kBlock / 64];
+    const uint32_t n = d_n ? min(*d_n, cap) : cap;
+    const uint32_t i = blockIdx.x * kBounceTile + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *d_count = totals[0];
+    const bool hit = i < n && prim[i] != kMiss;
+    uint32_t agg;
+    const uint32_t m = counts[blockIdx.x] + block_excl_scan<kBlock / 64>(hit ? 1u : 0u, &agg, s_scan);
+    if (!hit) return;
+    v3 no, nd;
+    bounce_ray(tris, rays, i, prim[i], tin[i], seed, no, nd);
+    tmp.a[0][m] = no.x;
+    tmp.a[1][m] = no.y;
+    tmp.a[2][m] = no.z;
+    tmp.a[3][m] = nd.x;
+    tmp.a[4][m] = nd.y;
+    tmp.a[5][m] = nd.z;
+    tmp.a[6][m] = 3.402823466e+38f;
+    keys[m] = bounce_key(q, no, nd);
+    vals[m] = m;
+    tsrc[m] = i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bounce_gather(RaySoAOut tmp, const uint32_t* __restrict__ vals,
+                                                          const uint32_t* __restrict__ tsrc, uint32_t cap,
+                                                          const uint32_t* d_count, RaySoAOut out,
+                                                          uint32_t* __restrict__ src) {
+    const uint32_t n = min(*d_count, cap);
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        const uint32_t m = vals[j];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) out.a[k][j] = tmp.a[k][m];
+        if (src) src[j] = tsrc[m];
+    }
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t bounce_ordered_scratch_bytes(uint32_t n) {
+    const size_t tiles = ((size_t)n + kBounceTile - 1) / kBounceTile;
+    const size_t w = align256((size_t)n * 4);
+    return align256((tiles + 16) * 4) + 7 * w + 5 * w + sort_scratch_words(n, 16, 8) * 4 + 256;
+}
+
+void launch_bounce_ordered(const TriSoA& tris, const RaySoA& r, const uint32_t* prim, const float* t, uint32_t n,
+                           const uint32_t* d_n, uint64_t seed, const BounceOrder& q, const RaySoAOut& out,
+                           uint32_t* src, uint32_t* d_count, void* scratch, hipStream_t st) {
+    if (n == 0) {
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, d_count, 0u);
+        return;
+    }
+    const uint32_t tiles = (n + kBounceTile - 1) / kBounceTile;
+    const size_t w = align256((size_t)n * 4);
+    char* p = reinterpret_cast<char*>(scratch);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(p);
+    uint32_t* totals = counts + tiles;
+    p += align256(((size_t)tiles + 16) * 4);
+    RaySoAOut tmp;
+    for (int k = 0; k < 7; ++k, p += w) tmp.a[k] = reinterpret_cast<float*>(p);
+    uint32_t* k0 = reinterpret_cast<uint32_t*>(p);
+    uint32_t* v0 = reinterpret_cast<uint32_t*>(p + w);
+    uint32_t* k1 = reinterpret_cast<uint32_t*>(p + 2 * w);
+    uint32_t* v1 = reinterpret_cast<uint32_t*>(p + 3 * w);
+    uint32_t* tsrc = reinterpret_cast<uint32_t*>(p + 4 * w);
+    uint32_t* sort_scratch = reinterpret_cast<uint32_t*>(p + 5 * w);
+    hipLaunchKernelGGL(k_bounce_hits, dim3(tiles), dim3(kBlock), 0, st, prim, n, d_n, counts);
+    hipLaunchKernelGGL(k_bounce_scan, dim3(1), dim3(1024), 0, st, counts, tiles, totals);
+    hipLaunchKernelGGL(k_bounce_keyed, dim3(tiles), dim3(kBlock), 0, st, tris, r, prim, t, n, d_n, seed, q, tmp, k0,
+                       v0, tsrc, counts, totals, d_count);
+    launch_sort_pairs(k0, v0, k1, v1, d_count, n, 16, 8, sort_scratch, st);
+    const uint32_t* vs = (sort_passes(16, 8) & 1) ? v1 : v0;
+    uint32_t gb = (n + kBlock - 1) / kBlock;
+    gb = gb > 4096 ? 4096 : gb;
+    hipLaunchKernelGGL(k_bounce_gather, dim3(gb), dim3(kBlock), 0, st, tmp, vs, tsrc, n, d_count, out, src);
+}
+
 __global__ void k_set_u32(uint32_t* p, uint32_t v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }

@@ -111,7 +111,21 @@ def test_served_mix_with_dynamic_mesh_matches_oracle(ctx, oracle):
     frames = [W.sphere_mesh(200_000, frame=f) for f in range(6)]
     K = 24
     tsz = _log_uniform(rng, 1_000, 1_000_000, K)
-    temps = [sa.gen_random_rays(900 + k, int(n), 0.0, 1.0) for k, n in enumerate(tsz)]
+    # request rays and results in pinned host memory, as a serving client
+    # keeps them (pageable buffers make every copy a staged, blocking one)
+    pinned = []          # owners of the pinned buffers, alive until the end
+    temps = []
+    for k, n in enumerate(tsz):
+        r = sa.PinnedArray(ctx, (7, int(n)), np.float32)
+        r.a[:] = sa.gen_random_rays(900 + k, int(n), 0.0, 1.0)
+        pinned.append(r)
+        temps.append(r.a)
+    outs = []
+
+    def out_for(n):
+        p, t = sa.PinnedArray(ctx, (n,), np.uint32), sa.PinnedArray(ctx, (n,), np.float32)
+        outs.append((p, t))
+        return p.a, t.a
 
     srv = sa.Server(ctx, mode="batched", hbm_budget=3 << 30, max_requests=512, max_waves=4, max_wave_rays=1 << 22)
     ids = [srv.add_grid(g) for _, _, g in statics]
@@ -138,6 +152,7 @@ def test_served_mix_with_dynamic_mesh_matches_oracle(ctx, oracle):
     gaps = rng.exponential(1.0 / 2000.0, n_max)
     which = rng.integers(0, K, n_max)
     scene_of = rng.integers(0, 17, n_max)
+    bufs = [out_for(int(tsz[which[k]])) for k in range(400)]
     th = threading.Thread(target=producer, daemon=True)
     th.start()
     rids = []
@@ -145,10 +160,14 @@ def test_served_mix_with_dynamic_mesh_matches_oracle(ctx, oracle):
     k = 0
     while k < n_max and (k < 400 or len(version_frame) < 5):
         t += gaps[k]
-        while time.perf_counter() < t:
-            pass
+        while True:        # sleep / yield, never spin holding the GIL: the mesh thread runs on time
+            left = t - time.perf_counter()
+            if left <= 0:
+                break
+            time.sleep(left - 1e-4 if left > 3e-4 else 0)
         sid = ids[scene_of[k]] if scene_of[k] < 16 else dyn
-        rids.append(srv.submit(sid, temps[which[k]]))
+        p_out, t_out = bufs[k] if k < len(bufs) else out_for(int(tsz[which[k]]))
+        rids.append(srv.submit(sid, temps[which[k]], p_out, t_out))
         k += 1
     n_req = k
     results = [srv.wait(r, with_version=True) for r in rids]

@@ -257,3 +257,71 @@ def test_device_count_small_batches(ctx, oracle, kind, shadow, count):
     np.testing.assert_array_equal(pg[:count], po)
     np.testing.assert_array_equal(t.download(np.float32, count).view(np.uint32), to.view(np.uint32))
     assert (pg[count:count + 4096] == 0xABCDEF01).all()
+
+
+ORDER_CASES = [
+    ("heightfield_exit", lambda: W.heightfield2m(grid=256, res_x=320, res_y=200), "exit"),
+    ("heightfield_axis", lambda: W.heightfield2m(grid=256, res_x=320, res_y=200), "axis"),
+    ("cluster_exit", lambda: W.cluster(n_small=60_000, n_large=400, res_x=320, res_y=180), "exit"),
+]
+
+
+@pytest.mark.parametrize("name,make,order", ORDER_CASES, ids=[c[0] for c in ORDER_CASES])
+def test_bounce_ordered_equals_oracle(ctx, oracle, name, make, order):
+    """sg_bounce_ordered: the rays, their sources and their count equal the
+    oracle's ordered bounce bit for bit (exit distance from the grid's
+    bounds, longest first; or |d . up| smallest first)"""
+    w = make()
+    scene = sa.FaceSoup(ctx, w.scene)
+    b = sa.FastTwoLevelGridBuilder(ctx, **w.build)
+    b.init(scene)
+    g = b.build()
+    rb = sa.RayBatch(ctx, w.rays)
+    prim_d, t_d = g.trace(rb)
+    ctx.sync()
+    prim, t = prim_d.download(np.uint32, rb.n), t_d.download(np.float32, rb.n)
+    info = g.info()
+    box = (tuple(info.bmin), tuple(info.bmax))
+    q = sa.bounce_order(order, grid=g, axis=(0.0, 0.0, 1.0))
+    out = sa.RayBatch(ctx, capacity=rb.n)
+    src = sa.DeviceBuffer(ctx, 4 * rb.n)
+    count = sa.bounce(ctx, scene, rb, prim_d, t_d, w.bounce_seed, out, src=src, order=q)
+    ctx.sync()
+    m = int(count.download(np.uint32, 1)[0])
+    b_o, src_o = oracle.bounce(w.scene, w.rays, prim, t, w.bounce_seed, order=order, box=box, axis=(0.0, 0.0, 1.0))
+    assert m == b_o.shape[1] > 1000
+    np.testing.assert_array_equal(out.download(m).view(np.uint32), b_o.view(np.uint32))
+    np.testing.assert_array_equal(src.download(np.uint32, m), src_o)
+    g.close()
+    scene.close()
+
+
+def test_paths_with_ordered_bounces_equal_oracle(ctx, oracle):
+    """four path segments chained on the device with exit-ordered bounces
+    (counts never read back): every segment's rays, ids and t equal the
+    oracle's"""
+    w = W.cluster(n_small=60_000, n_large=400, res_x=128, res_y=72)
+    scene = sa.FaceSoup(ctx, w.scene)
+    b = sa.FastTwoLevelGridBuilder(ctx, **w.build)
+    b.init(scene)
+    g = b.build()
+    info = g.info()
+    box = (tuple(info.bmin), tuple(info.bmax))
+    rb = sa.RayBatch(ctx, w.rays)
+    counts, segs = sa.trace_paths(ctx, scene, g, rb, depth=4, seed=w.bounce_seed, keep=True,
+                                  order=sa.bounce_order("exit", box=box))
+    ctx.sync()
+    og = oracle.build_twolevel(w.scene, **w.oracle)
+    rays_o = w.rays
+    for d, (rays_d, p, t, cnt) in enumerate(segs):
+        n = rays_o.shape[1]
+        if d:
+            assert int(counts[d - 1].download(np.uint32, 1)[0]) == n
+        np.testing.assert_array_equal(rays_d.download(n).view(np.uint32), rays_o.view(np.uint32), err_msg=f"seg {d}")
+        po, to = og.trace(rays_o)
+        np.testing.assert_array_equal(p.download(np.uint32, n), po, err_msg=f"seg {d}")
+        np.testing.assert_array_equal(t.download(np.float32, n).view(np.uint32), to.view(np.uint32))
+        if d + 1 < len(segs):
+            rays_o, _ = oracle.bounce(w.scene, rays_o, po, to, w.bounce_seed + d, order="exit", box=box)
+    g.close()
+    scene.close()

@@ -259,3 +259,38 @@ def test_trace_store_geometry(oracle, scene, rays, kind):
     nrm /= np.linalg.norm(nrm, axis=0)
     plane = np.abs(np.sum((P + 1e-5 * D[:, hit] - v[0]) * nrm, axis=0))
     assert np.max(plane) < 2e-6
+
+
+@pytest.mark.parametrize("order", ["exit", "axis"])
+def test_bounce_ordered(oracle, scene, order):
+    """the ordered bounce rays are the class-round bounce rays, permuted: the
+    same (source, ray) pairs; the float64 exit distance from the box
+    (longest first) or |d . axis| (smallest first) of consecutive rays runs
+    monotone up to one 16-bit step (1/65535 of the box diagonal, or of 1),
+    and rays in one step keep input order"""
+    r = sa.gen_random_rays(6, 3000, 0.0, 1.0)
+    g = oracle.build_twolevel(scene, lambda_leaf=2.0)
+    prim, t = g.trace(r)
+    b0, s0 = oracle.bounce(scene, r, prim, t, 7)
+    box = (tuple(g.bmin), tuple(g.bmax))
+    axis = (0.0, 0.6, 0.8)
+    b1, s1 = oracle.bounce(scene, r, prim, t, 7, order=order, box=box, axis=axis)
+    assert b1.shape == b0.shape and b1.shape[1] > 500
+    p0, p1 = np.argsort(s0), np.argsort(s1)
+    np.testing.assert_array_equal(s0[p0], s1[p1])
+    np.testing.assert_array_equal(b0[:, p0].view(np.uint32), b1[:, p1].view(np.uint32))
+    o, d = b1[0:3].astype(np.float64), b1[3:6].astype(np.float64)
+    if order == "exit":
+        lo, hi = np.array(box[0])[:, None], np.array(box[1])[:, None]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            far = np.fmax((lo - o) / d, (hi - o) / d)
+        step = float(np.linalg.norm(np.array(box[1]) - np.array(box[0]))) / 65535.0
+        key = -np.maximum(np.min(far, axis=0), 0.0) / step    # ascending = longest first, in steps
+    else:
+        key = np.abs(np.sum(d * np.array(axis)[:, None], axis=0)) * 65535.0
+    dk = np.diff(key)
+    assert (dk >= -1.0).all(), dk.min()                       # monotone to within one step
+    assert (dk > 2.0).sum() > 100                              # and really ordered
+    bucket = np.floor(key + 1e-9)
+    same = np.abs(np.diff(key)) < 1e-3                         # clearly one step: input order
+    assert (s1[1:][same] > s1[:-1][same]).all()
