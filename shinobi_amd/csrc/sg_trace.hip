@@ -1062,11 +1062,14 @@ __global__ __launch_bounds__(kBlock) void k_bounce_hits(const uint32_t* __restri
     if (threadIdx.x == 0) counts[blockIdx.x] = c;
 }
 
+// The compacted rays wait for the sort as one 32-byte record each,
+// {o.x, o.y, o.z, d.x} {d.y, d.z, source index, 0}, so the gather reads one
+// half cache line per ray instead of a line from each of seven arrays.
 __global__ __launch_bounds__(kBlock) void k_bounce_keyed(TriSoA tris, RaySoA rays, const uint32_t* __restrict__ prim,
                                                          const float* __restrict__ tin, uint32_t cap,
                                                          const uint32_t* d_n, uint64_t seed, BounceOrder q,
-                                                         RaySoAOut tmp, uint32_t* __restrict__ keys,
-                                                         uint32_t* __restrict__ vals, uint32_t* __restrict__ tsrc,
+                                                         float4* __restrict__ rec, uint32_t* __restrict__ keys,
+                                                         uint32_t* __restrict__ vals,
                                                          const uint32_t* __restrict__ counts,
                                                          const uint32_t* __restrict__ totals, uint32_t* d_count) {
     __shared__ uint32_t s_scan[kBlock / 64];
@@ -1079,28 +1082,28 @@ __global__ __launch_bounds__(kBlock) void k_bounce_keyed(TriSoA tris, RaySoA ray
     if (!hit) return;
     v3 no, nd;
     bounce_ray(tris, rays, i, prim[i], tin[i], seed, no, nd);
-    tmp.a[0][m] = no.x;
-    tmp.a[1][m] = no.y;
-    tmp.a[2][m] = no.z;
-    tmp.a[3][m] = nd.x;
-    tmp.a[4][m] = nd.y;
-    tmp.a[5][m] = nd.z;
-    tmp.a[6][m] = 3.402823466e+38f;
+    rec[2 * (size_t)m] = make_float4(no.x, no.y, no.z, nd.x);
+    rec[2 * (size_t)m + 1] = make_float4(nd.y, nd.z, __uint_as_float(i), 0.f);
     keys[m] = bounce_key(q, no, nd);
     vals[m] = m;
-    tsrc[m] = i;
 }
 
-__global__ __launch_bounds__(kBlock) void k_bounce_gather(RaySoAOut tmp, const uint32_t* __restrict__ vals,
-                                                          const uint32_t* __restrict__ tsrc, uint32_t cap,
+__global__ __launch_bounds__(kBlock) void k_bounce_gather(const float4* __restrict__ rec,
+                                                          const uint32_t* __restrict__ vals, uint32_t cap,
                                                           const uint32_t* d_count, RaySoAOut out,
                                                           uint32_t* __restrict__ src) {
     const uint32_t n = min(*d_count, cap);
     for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
         const uint32_t m = vals[j];
-#pragma unroll
-        for (int k = 0; k < 7; ++k) out.a[k][j] = tmp.a[k][m];
-        if (src) src[j] = tsrc[m];
+        const float4 a = rec[2 * (size_t)m], b = rec[2 * (size_t)m + 1];
+        out.a[0][j] = a.x;
+        out.a[1][j] = a.y;
+        out.a[2][j] = a.z;
+        out.a[3][j] = a.w;
+        out.a[4][j] = b.x;
+        out.a[5][j] = b.y;
+        out.a[6][j] = 3.402823466e+38f;
+        if (src) src[j] = __float_as_uint(b.z);
     }
 }
 
@@ -1109,7 +1112,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 size_t bounce_ordered_scratch_bytes(uint32_t n) {
     const size_t tiles = ((size_t)n + kBounceTile - 1) / kBounceTile;
     const size_t w = align256((size_t)n * 4);
-    return align256((tiles + 16) * 4) + 7 * w + 5 * w + sort_scratch_words(n, 16, 8) * 4 + 256;
+    return align256((tiles + 16) * 4) + 8 * w + 4 * w + sort_scratch_words(n, 16, 8) * 4 + 256;
 }
 
 void launch_bounce_ordered(const TriSoA& tris, const RaySoA& r, const uint32_t* prim, const float* t, uint32_t n,
@@ -1125,23 +1128,22 @@ void launch_bounce_ordered(const TriSoA& tris, const RaySoA& r, const uint32_t* 
     uint32_t* counts = reinterpret_cast<uint32_t*>(p);
     uint32_t* totals = counts + tiles;
     p += align256(((size_t)tiles + 16) * 4);
-    RaySoAOut tmp;
-    for (int k = 0; k < 7; ++k, p += w) tmp.a[k] = reinterpret_cast<float*>(p);
+    float4* rec = reinterpret_cast<float4*>(p);   // 32 bytes per ray: 8 words
+    p += 8 * w;
     uint32_t* k0 = reinterpret_cast<uint32_t*>(p);
     uint32_t* v0 = reinterpret_cast<uint32_t*>(p + w);
     uint32_t* k1 = reinterpret_cast<uint32_t*>(p + 2 * w);
     uint32_t* v1 = reinterpret_cast<uint32_t*>(p + 3 * w);
-    uint32_t* tsrc = reinterpret_cast<uint32_t*>(p + 4 * w);
-    uint32_t* sort_scratch = reinterpret_cast<uint32_t*>(p + 5 * w);
+    uint32_t* sort_scratch = reinterpret_cast<uint32_t*>(p + 4 * w);
     hipLaunchKernelGGL(k_bounce_hits, dim3(tiles), dim3(kBlock), 0, st, prim, n, d_n, counts);
     hipLaunchKernelGGL(k_bounce_scan, dim3(1), dim3(1024), 0, st, counts, tiles, totals);
-    hipLaunchKernelGGL(k_bounce_keyed, dim3(tiles), dim3(kBlock), 0, st, tris, r, prim, t, n, d_n, seed, q, tmp, k0,
-                       v0, tsrc, counts, totals, d_count);
+    hipLaunchKernelGGL(k_bounce_keyed, dim3(tiles), dim3(kBlock), 0, st, tris, r, prim, t, n, d_n, seed, q, rec, k0,
+                       v0, counts, totals, d_count);
     launch_sort_pairs(k0, v0, k1, v1, d_count, n, 16, 8, sort_scratch, st);
     const uint32_t* vs = (sort_passes(16, 8) & 1) ? v1 : v0;
     uint32_t gb = (n + kBlock - 1) / kBlock;
     gb = gb > 4096 ? 4096 : gb;
-    hipLaunchKernelGGL(k_bounce_gather, dim3(gb), dim3(kBlock), 0, st, tmp, vs, tsrc, n, d_count, out, src);
+    hipLaunchKernelGGL(k_bounce_gather, dim3(gb), dim3(kBlock), 0, st, rec, vs, n, d_count, out, src);
 }
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) {
