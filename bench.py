@@ -113,6 +113,21 @@ def host_cpu() -> dict:
             "cores": min(cores, quota) if quota else cores}
 
 
+def counters_of(launch: str) -> dict:
+    """VALU / lane / L2 counters of a traversal launch from the committed
+    rocprofv3 SQ / TCC summary of this workload ({} when it does not list it)"""
+    path = os.path.join(ROOT, "profiles", "heightfield2m_pmc_traversal.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get(launch)
+    except (OSError, ValueError, KeyError):
+        return {}
+    if not k:
+        return {}
+    return {"valu_util": k.get("valu_util"), "lanes_active": k.get("lanes_active"), "l2_hit": k.get("l2_hit"),
+            "waves_per_simd": k.get("waves_per_simd")}
+
+
 def cpu_baseline(w, n_primary: int, seed: int, threads: int, runs: int = 5) -> dict:
     """The oracle (the restatement of the reference's GPU path, compiled
     -O2 for the host) on this host's cores: two-level build, closest-hit
@@ -565,13 +580,16 @@ def main() -> int:
     for name, f, ms in (("primary", f_primary, mean(tp_ms)), ("bounce", f_bounce, mean(tb_ms))):
         tf = f / (ms * 1e-3) / 1e12
         launches[name] = {"ms": round(ms, 4), "flops": f, "achieved": round(tf, 3),
-                          "frac": round(tf / FP32_PEAK_TFLOPS, 4), "traffic": traffic_of(f"k_trace {name}")}
+                          "frac": round(tf / FP32_PEAK_TFLOPS, 4), "traffic": traffic_of(f"k_trace {name}"),
+                          **counters_of(f"k_trace {name}")}
     dom = max(launches, key=lambda k: launches[k]["ms"])
     dominant = max([("trace_primary", mean(tp_ms)), ("trace_bounce", mean(tb_ms))] +
                    [(p, mean(v)) for p, v in ph.items()], key=lambda x: x[1])
     roofline = {"kernel": kernel_of[dom], "bound": "valu",
                 "achieved": launches[dom]["achieved"], "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": launches[dom]["frac"], "traffic": launches[dom]["traffic"],
+                "valu_util": launches[dom].get("valu_util"), "lanes_active": launches[dom].get("lanes_active"),
+                "l2_hit": launches[dom].get("l2_hit"),
                 "flops_per_launch": launches[dom]["flops"], "launches": launches,
                 "dominant_phase": dominant[0]}
     roofline_build = {"bound": "hbm", "achieved": round(build_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
