@@ -355,13 +355,18 @@ def ruins_bench(sa, W, ctx, reps: int = 10) -> dict:
             "mrays_per_s": round((n + m) / (r[1] + r[3]) / 1e3, 1)}
 
 
+BATCH_BRANCHES = 16   # concurrent rebuild branches of the batched graph
+
+
 def batched_bench(sa, ctx, n_scenes: int = 64, reps: int = 5) -> dict:
     """Batched multi-scene launch: 64 uniform-in-cube soups, N log-uniform in
     [1e3, 5e5] with edge scale 0.01*N^(-1/3), each with log-uniform
     [1e4, 2e6] random rays; per-scene density, radix width and block size
     from the autotuner.  One captured graph rebuilds every grid and traces
-    every batch in one multi-batch launch; the plain path runs the same
-    rebuilds and one trace launch per scene."""
+    every batch in one multi-batch launch, the rebuilds on concurrent
+    branches; the plain path runs the same rebuilds and one trace launch per
+    scene.  graph_serial_rebuilds_ms is the graph with the rebuilds in
+    sequence."""
     rng = np.random.default_rng(4)
     ns = _log_uniform(rng, 1_000, 500_000, n_scenes)
     nr = _log_uniform(rng, 10_000, 2_000_000, n_scenes)
@@ -392,7 +397,12 @@ def batched_bench(sa, ctx, n_scenes: int = 64, reps: int = 5) -> dict:
         batch.add(g, rays, prim, tb)
     batch.commit()
     ctx.sync()
+    # the rebuilds run on concurrent branches (sg_rebuild_many), the traces
+    # in one multi-batch launch after the join
     with sa.Graph.capture(ctx) as graph:
+        batch.rebuild(branches=BATCH_BRANCHES)
+        batch.launch()
+    with sa.Graph.capture(ctx) as serial:
         for soup, g, rays, prim, tb in items:
             g.rebuild()
         batch.launch()
@@ -420,6 +430,7 @@ def batched_bench(sa, ctx, n_scenes: int = 64, reps: int = 5) -> dict:
             g.trace(rays, prim=prim, t=tb)
 
     ms_graph = timeit(graph.launch)
+    ms_serial = timeit(serial.launch)
     ms_plain = timeit(plain)
     ms_batch_trace = timeit(lambda: batch.launch())
     ms_plain_trace = timeit(plain_trace)
@@ -430,10 +441,12 @@ def batched_bench(sa, ctx, n_scenes: int = 64, reps: int = 5) -> dict:
     same = all(np.array_equal(p.download(np.uint32, r.n), q) for (_, _, r, p, _), q in zip(items, ref))
     total_rays = int(nr.sum())
     graph.close()
+    serial.close()
     return {"scenes": n_scenes, "tris_total": int(ns.sum()), "rays_total": total_rays,
             "autotune_s": round(t_tune, 1), "chosen": {"two_level": kinds[1], "uniform": kinds[0],
                                                        "radix_bits": radix, "block": blocks},
-            "graph_ms": round(ms_graph, 3), "plain_ms": round(ms_plain, 3),
+            "rebuild_branches": BATCH_BRANCHES, "graph_ms": round(ms_graph, 3),
+            "graph_serial_rebuilds_ms": round(ms_serial, 3), "plain_ms": round(ms_plain, 3),
             "batched_trace_ms": round(ms_batch_trace, 3), "plain_trace_ms": round(ms_plain_trace, 3),
             "graph_mrays_per_s": round(total_rays / ms_graph / 1e3, 1),
             "plain_mrays_per_s": round(total_rays / ms_plain / 1e3, 1),

@@ -150,6 +150,19 @@ sg_status sg_build(sg_ctx *ctx, sg_scene *scene, const sg_build_params *p, void 
 sg_status sg_rebuild(sg_ctx *ctx, sg_scene *scene, sg_grid *grid, void *stream);
 /* like sg_rebuild but lets the device recompute the resolution as well */
 sg_status sg_rebuild_free_res(sg_ctx *ctx, sg_scene *scene, sg_grid *grid, void *stream);
+/* sg_rebuild of n (scene, grid) pairs, the rebuilds spread over up to
+ * `branches` (1..SG_REBUILD_BRANCHES_MAX, 0 = the maximum) streams the
+ * context owns: the branches start after the work already enqueued on
+ * `stream` and the work enqueued on `stream` afterwards waits for all of
+ * them.  Largest scene first, each to the branch with the least work so far.
+ * A small scene's rebuild is a dozen short launches that leave most of the
+ * device idle; concurrent branches overlap them.  Captures into a hipGraph
+ * as parallel branches.  Grids must be distinct; each grid's result equals
+ * its sg_rebuild.  No counterpart in the reference, whose builders run one
+ * scene at a time (FastTwoLevelGridBuilder.h:755-781). */
+#define SG_REBUILD_BRANCHES_MAX 16
+sg_status sg_rebuild_many(sg_ctx *ctx, uint32_t n, sg_scene *const *scenes, sg_grid *const *grids,
+                          uint32_t branches, void *stream);
 /* sg_rebuild that also records hipEvent_t events[i] at the phase boundaries,
  * the analogue of the builders' phase timers (FastTwoLevelGridBuilder.h:
  * 755-781): [0] start, [1] bounds, [2] top emission, [3] top sort, [4] top
@@ -335,6 +348,9 @@ sg_status sg_event_create(sg_ctx *ctx, void **out);
 sg_status sg_event_destroy(sg_ctx *ctx, void *ev);
 sg_status sg_event_record(sg_ctx *ctx, void *ev, void *stream);
 sg_status sg_event_elapsed_ms(sg_ctx *ctx, void *ev_start, void *ev_end, float *ms);
+/* work enqueued on `stream` after this call waits for `ev` (a fork / join
+ * between streams; under graph capture it becomes a graph dependency) */
+sg_status sg_stream_wait_event(sg_ctx *ctx, void *stream, void *ev);
 
 /* ------------------------------------------------------------ batching --- */
 /* Many (grid, ray batch) items traced by one launch per grid kind: the

@@ -37,7 +37,7 @@ __all__ = [
     "Context", "DeviceBuffer", "FaceSoup", "FastSimpleGridBuilder", "FastTwoLevelGridBuilder", "Grid",
     "GridInfo", "RayBatch", "ShinobiError", "MISS", "FLT_MAX", "gen_soup", "gen_heightfield",
     "gen_cluster", "gen_camera_rays", "gen_random_rays", "heightfield_height", "build_params",
-    "device_count", "lib", "load_obj", "load_camera",
+    "device_count", "lib", "load_obj", "load_camera", "Batch", "Graph", "autotune", "rebuild_many",
 ]
 
 
@@ -358,9 +358,9 @@ class Grid:
               f"sg_grid_download({which})")
         return out[: sizes[which]]
 
-    def rebuild(self, free_resolution: bool = False) -> None:
+    def rebuild(self, free_resolution: bool = False, stream=None) -> None:
         f = self.ctx._l.sg_rebuild_free_res if free_resolution else self.ctx._l.sg_rebuild
-        check(f(self.ctx.h, self.scene.h, self.h, self.ctx.stream), "sg_rebuild")
+        check(f(self.ctx.h, self.scene.h, self.h, self.ctx.stream if stream is None else stream), "sg_rebuild")
 
     def rebuild_phased(self, events: list["Event"]) -> None:
         """rebuild() recording events at the phase boundaries (see
@@ -560,6 +560,18 @@ def sort_pairs(ctx: Context, keys: np.ndarray, vals: np.ndarray, key_bits: int =
 
 
 # ------------------------------------------------------------------ batching
+def rebuild_many(ctx: Context, grids: list, branches: int = 0, stream=None) -> None:
+    """Rebuild each grid from its scene (Grid.rebuild for all of them),
+    spread over up to `branches` concurrent streams of the context (0: the
+    maximum, SG_REBUILD_BRANCHES_MAX) and joined back into `stream`
+    (sg_rebuild_many); captures into a Graph as parallel branches."""
+    n = len(grids)
+    scenes = (C.c_void_p * max(n, 1))(*[g.scene.h.value for g in grids])
+    hs = (C.c_void_p * max(n, 1))(*[g.h.value for g in grids])
+    check(ctx._l.sg_rebuild_many(ctx.h, n, scenes, hs, int(branches), ctx.stream if stream is None else stream),
+          "sg_rebuild_many")
+
+
 class Batch:
     """Many (grid, ray batch) items traced by one launch per grid kind
     (sg_batch_*)."""
@@ -570,16 +582,25 @@ class Batch:
         check(ctx._l.sg_batch_create(ctx.h, max_items, C.byref(self.h)), "sg_batch_create")
         ctx._own(self)
         self._keep: list = []
+        self._grids: list = []
 
     def clear(self) -> None:
         check(self.ctx._l.sg_batch_clear(self.h), "sg_batch_clear")
         self._keep = []
+        self._grids = []
 
     def add(self, grid: "Grid", rays: RayBatch, prim: DeviceBuffer, t: DeviceBuffer, n: int | None = None) -> None:
         r = rays.struct()
         check(self.ctx._l.sg_batch_add(self.h, grid.h, C.byref(r), rays.n if n is None else n, prim.ptr, t.ptr),
               "sg_batch_add")
         self._keep += [grid, rays, prim, t]
+        if all(g is not grid for g in self._grids):
+            self._grids.append(grid)
+
+    def rebuild(self, branches: int = 0, stream=None) -> None:
+        """Rebuild every grid of the batch from its scene's current vertices
+        on concurrent branches (sg_rebuild_many)."""
+        rebuild_many(self.ctx, self._grids, branches, stream)
 
     def commit(self, stream=None) -> None:
         check(self.ctx._l.sg_batch_commit(self.h, self.ctx.stream if stream is None else stream), "sg_batch_commit")

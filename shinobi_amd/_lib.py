@@ -20,6 +20,7 @@ STATUS_NAMES = {
     12: "SG_ERR_IO", 13: "SG_ERR_SCRATCH_LIMIT",
 }
 SG_ERR_SCRATCH_LIMIT = 13
+SG_REBUILD_BRANCHES_MAX = 16
 SG_BOUNCE_NOALLOC_MAX_RAYS = 16777216
 
 SG_GRID_UNIFORM = 0
@@ -174,6 +175,9 @@ SIGNATURES = {
     "sg_event_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "sg_event_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sg_event_record": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sg_rebuild_many": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_uint32,
+                                  C.c_void_p]),
+    "sg_stream_wait_event": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "sg_event_elapsed_ms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     "sg_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
     "sg_batch_destroy": (C.c_int, [C.c_void_p]),

@@ -103,6 +103,14 @@ sg_status sg_ctx_create(int device, sg_ctx** out) {
     if (ns != SG_OK) return ns;
     sg_ctx* c = new sg_ctx();
     c->device = device;
+    bool ok = hipEventCreateWithFlags(&c->fork_start, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; ok && k < SG_REBUILD_BRANCHES_MAX; ++k)
+        ok = hipStreamCreateWithFlags(&c->fork_st[k], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&c->fork_join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        sg_ctx_destroy(c);
+        return SG_ERR_HIP;
+    }
     *out = c;
     return SG_OK;
 }
@@ -113,6 +121,11 @@ sg_status sg_ctx_destroy(sg_ctx* ctx) {
     if (!ctx->scratch.empty() || !ctx->retired.empty()) hipDeviceSynchronize();
     for (auto& kv : ctx->scratch) hipFree(kv.second.p);
     for (void* p : ctx->retired) hipFree(p);
+    for (int k = 0; k < SG_REBUILD_BRANCHES_MAX; ++k) {
+        if (ctx->fork_st[k]) hipStreamDestroy(ctx->fork_st[k]);
+        if (ctx->fork_join[k]) hipEventDestroy(ctx->fork_join[k]);
+    }
+    if (ctx->fork_start) hipEventDestroy(ctx->fork_start);
     delete ctx;
     return SG_OK;
 }
@@ -505,6 +518,63 @@ sg_status sg_rebuild(sg_ctx* ctx, sg_scene* scene, sg_grid* g, void* stream) {
 
 sg_status sg_rebuild_free_res(sg_ctx* ctx, sg_scene* scene, sg_grid* g, void* stream) {
     return rebuild_impl(ctx, scene, g, stream, false);
+}
+
+// Fixed cost of one rebuild in triangles: a rebuild is about a dozen
+// launches of a few microseconds each, which a scene of this size takes in
+// work alone.  Weighs the branch assignment of sg_rebuild_many.
+static constexpr uint64_t kRebuildLaunchTris = 50000;
+
+sg_status sg_rebuild_many(sg_ctx* ctx, uint32_t n, sg_scene* const* scenes, sg_grid* const* grids, uint32_t branches,
+                          void* stream) {
+    if (!ctx || branches > SG_REBUILD_BRANCHES_MAX) return SG_ERR_INVALID_ARG;
+    if (n == 0) return SG_OK;
+    if (!scenes || !grids) return SG_ERR_INVALID_ARG;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!scenes[i] || !grids[i] || scenes[i]->n != grids[i]->n_tris) return SG_ERR_INVALID_ARG;
+        for (uint32_t j = 0; j < i; ++j)
+            if (grids[j] == grids[i]) return SG_ERR_INVALID_ARG;
+    }
+    const uint32_t K = std::min(branches ? branches : (uint32_t)SG_REBUILD_BRANCHES_MAX, n);
+    if (K == 1) {
+        for (uint32_t i = 0; i < n; ++i) {
+            const sg_status r = rebuild_impl(ctx, scenes[i], grids[i], stream, true);
+            if (r != SG_OK) return r;
+        }
+        return SG_OK;
+    }
+    // largest first, each to the branch with the least work so far
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return grids[a]->n_tris > grids[b]->n_tris; });
+    std::vector<uint64_t> load(K, 0);
+    std::vector<uint32_t> branch(n);
+    for (uint32_t i : order) {
+        const uint32_t k = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        branch[i] = k;
+        load[k] += grids[i]->n_tris + kRebuildLaunchTris;
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> lk(ctx->fork_mu);
+    const hipStream_t st = (hipStream_t)stream;
+    HIPCHK(hipEventRecord(ctx->fork_start, st));
+    for (uint32_t k = 0; k < K; ++k) HIPCHK(hipStreamWaitEvent(ctx->fork_st[k], ctx->fork_start, 0));
+    sg_status r = SG_OK;
+    for (uint32_t i : order) {
+        if (r == SG_OK) r = rebuild_impl(ctx, scenes[i], grids[i], ctx->fork_st[branch[i]], true);
+        // the grid's stream (synchronised before its header is read) is the
+        // caller's: the join below orders the rebuild before its later work,
+        // and graph replays never run on the side streams
+        grids[i]->stream = st;
+    }
+    // join every branch, also after an error, so that no branch is left
+    // outside a capture
+    for (uint32_t k = 0; k < K; ++k) {
+        HIPCHK(hipEventRecord(ctx->fork_join[k], ctx->fork_st[k]));
+        HIPCHK(hipStreamWaitEvent(st, ctx->fork_join[k], 0));
+    }
+    return r;
 }
 
 sg_status sg_rebuild_phased(sg_ctx* ctx, sg_scene* scene, sg_grid* g, void* stream, void* const* events,
@@ -921,6 +991,13 @@ sg_status sg_event_destroy(sg_ctx* ctx, void* ev) {
 sg_status sg_event_record(sg_ctx* ctx, void* ev, void* stream) {
     if (!ctx || !ev) return SG_ERR_INVALID_ARG;
     HIPCHK(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+    return SG_OK;
+}
+
+sg_status sg_stream_wait_event(sg_ctx* ctx, void* stream, void* ev) {
+    if (!ctx || !ev) return SG_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)ev, 0));
     return SG_OK;
 }
 

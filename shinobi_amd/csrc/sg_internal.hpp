@@ -30,6 +30,12 @@ struct sg_ctx {
     std::mutex scratch_mu;
     std::map<hipStream_t, ScratchBuf> scratch;   // per stream (nullptr = the null stream)
     std::vector<void*> retired;
+    // side streams and events of sg_rebuild_many (created with the context,
+    // so that a call under graph capture creates nothing)
+    std::mutex fork_mu;
+    hipStream_t fork_st[SG_REBUILD_BRANCHES_MAX] = {};
+    hipEvent_t fork_join[SG_REBUILD_BRANCHES_MAX] = {};
+    hipEvent_t fork_start = nullptr;
 };
 
 struct sg_scene {

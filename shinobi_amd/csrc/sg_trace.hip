@@ -350,47 +350,56 @@ SG_D uint2 load_cell(const GView& g, const GParams& p, v3 c) {
     return ld2(g.cells + 2 * (size_t)k);
 }
 
+// one iteration of the uniform walk for every lane of the wave
+template <bool SHADOW>
+SG_D void uniform_iter(const GView& g, const GParams& p, Ray& r, Walk& w, Refs& q, uint32_t cap, uint32_t& steps) {
+    refs_coop(g, q, r, coop_threshold(w.active));
+    const bool test = q.it < q.e;
+    RefRec x;
+    if (test) ref_load(g, q, x);
+    // A lane whose cell ends in this iteration (its last test, or none)
+    // takes the DDA step before the test: the step does not depend on the
+    // test, and the next cell's range then loads while the record is in
+    // flight instead of after it.  The t check that decides whether the
+    // step stands uses the t after the test.
+    const bool fin = w.active && q.it + 1u >= q.e;
+    float tm = 0.f;
+    bool stepped = false;
+    uint2 nr = make_uint2(0u, 0u);
+    if (fin) {
+        tm = tmin(w.tMax);
+        stepped = dda_step(min_axis(w.tMax), w.cell, w.tMax, r.rcp, p.cs, p.resf) && w.guard++ < cap;
+        if (stepped) nr = load_cell(g, p, w.cell);
+    }
+    if (test) {
+        mt_test(x.a, x.b, x.c, q.it, r);
+        ++q.it;
+    }
+    if (fin) {
+        // the cell is done: t check, then the step taken above
+        if (SHADOW ? !(r.t >= kShadowT) : !(r.t > tm)) {
+            w.active = false;
+        } else {
+            ++steps;
+            w.active = stepped;
+            if (stepped) refs_begin(g, q, nr, r);
+        }
+    }
+}
+
+// the first cell of a walk: the guard counts it, its range starts the tests
+SG_D void uniform_enter(const GView& g, const GParams& p, Ray& r, Walk& w, Refs& q, uint32_t cap) {
+    if (w.active && !(w.guard++ < cap)) w.active = false;
+    if (w.active) refs_begin(g, q, load_cell(g, p, w.cell), r);
+}
+
 template <bool SHADOW>
 SG_D void walk_uniform(const GView& g, const GParams& p, Ray& r, Walk& w, uint32_t& steps) {
     const uint32_t cap = step_cap(p);
     Refs q;
     q.it = q.e = 0;
-    if (w.active && !(w.guard++ < cap)) w.active = false;
-    if (w.active) refs_begin(g, q, load_cell(g, p, w.cell), r);
-    while (__ballot(w.active)) {
-        refs_coop(g, q, r, coop_threshold(w.active));
-        const bool test = q.it < q.e;
-        RefRec x;
-        if (test) ref_load(g, q, x);
-        // A lane whose cell ends in this iteration (its last test, or none)
-        // takes the DDA step before the test: the step does not depend on
-        // the test, and the next cell's range then loads while the record
-        // is in flight instead of after it.  The t check that decides
-        // whether the step stands uses the t after the test.
-        const bool fin = w.active && q.it + 1u >= q.e;
-        float tm = 0.f;
-        bool stepped = false;
-        uint2 nr = make_uint2(0u, 0u);
-        if (fin) {
-            tm = tmin(w.tMax);
-            stepped = dda_step(min_axis(w.tMax), w.cell, w.tMax, r.rcp, p.cs, p.resf) && w.guard++ < cap;
-            if (stepped) nr = load_cell(g, p, w.cell);
-        }
-        if (test) {
-            mt_test(x.a, x.b, x.c, q.it, r);
-            ++q.it;
-        }
-        if (fin) {
-            // the cell is done: t check, then the step taken above
-            if (SHADOW ? !(r.t >= kShadowT) : !(r.t > tm)) {
-                w.active = false;
-            } else {
-                ++steps;
-                w.active = stepped;
-                if (stepped) refs_begin(g, q, nr, r);
-            }
-        }
-    }
+    uniform_enter(g, p, r, w, q, cap);
+    while (__ballot(w.active)) uniform_iter<SHADOW>(g, p, r, w, q, cap, steps);
 }
 
 // TwoLevelGridTraverser::traverse, src/Control/Algorithm/TwoLevelGridTraverser.h:445-671;

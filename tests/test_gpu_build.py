@@ -198,3 +198,70 @@ def test_invalid_scenes_are_reported(ctx):
     with pytest.raises(sa.ShinobiError) as e:
         b.build()
     assert e.value.status == 2
+
+
+def test_rebuild_many_graph_matches_oracle(ctx, oracle):
+    """sg_rebuild_many over six grids of both kinds, captured into a graph
+    (concurrent branches) and replayed: after every replay each grid equals
+    the oracle's build of its moved vertices at the grid's resolution, and
+    the traversal after the join reads the rebuilt grids"""
+    specs = [(0, 3_000, 81), (1, 40_000, 82), (0, 12_000, 83), (1, 2_000, 84), (1, 25_000, 85), (0, 80_000, 86)]
+    items = []
+    for kind, n, seed in specs:
+        a = sa.gen_soup(seed, n, 0.01)
+        kw = dict(lam=5.0, lambda_leaf=2.0) if kind else dict(lam=3.0)
+        scene, g = _gpu_build(ctx, a, kind, **kw)
+        moved = a.copy()
+        for j in (2, 5, 8):
+            moved[j] *= np.float32(1.1)
+        items.append((kind, kw, scene, g, a, moved, tuple(g.info().res)))
+    for kind, kw, scene, g, a, moved, res in items:
+        scene.upload_vertices(moved)
+    rays = sa.gen_random_rays(87, 20_000, 0.0, 1.0)
+    rb = sa.RayBatch(ctx, rays)
+    outs = [(sa.DeviceBuffer(ctx, 4 * rb.n), sa.DeviceBuffer(ctx, 4 * rb.n)) for _ in items]
+    batch = sa.Batch(ctx, len(items))
+    for it, (p, t) in zip(items, outs):
+        batch.add(it[3], rb, p, t)
+    batch.commit()
+    ctx.sync()
+    with sa.Graph.capture(ctx) as graph:
+        batch.rebuild(branches=4)
+        batch.launch()
+    expect = []
+    for kind, kw, scene, g, a, moved, res in items:
+        if kind:
+            og = oracle.build_twolevel(moved, top_divisor=6, res_override=res, **kw)
+        else:
+            og = oracle.build_uniform(moved, top_divisor=1, res_override=res, **kw)
+        assert og.status == 0
+        expect.append(og)
+    for rep in range(2):
+        # eager work between replays: grid 1 rebuilt from the original
+        # vertices, the outputs poisoned
+        items[1][2].upload_vertices(items[1][4])
+        items[1][3].rebuild()
+        items[1][2].upload_vertices(items[1][5])
+        for p, _ in outs:
+            p.upload(np.full(rb.n, 7, np.uint32))
+        graph.launch()
+        ctx.sync()
+        for (kind, kw, scene, g, a, moved, res), og, (p, t) in zip(items, expect, outs):
+            assert g.info().status == 0 and tuple(g.info().res) == res
+            _assert_top_level(g, og)
+            if kind:
+                _assert_leaf_level(g, og)
+            else:
+                np.testing.assert_array_equal(g.download("indices"), og.indices)
+            po, to = og.trace(rays)
+            np.testing.assert_array_equal(p.download(np.uint32, rb.n), po, err_msg=f"replay {rep}")
+            np.testing.assert_array_equal(t.download(np.float32, rb.n).view(np.uint32), to.view(np.uint32))
+    graph.close()
+    # eager form, and the argument checks
+    sa.rebuild_many(ctx, [it[3] for it in items], branches=1)
+    sa.rebuild_many(ctx, [], branches=4)
+    with pytest.raises(sa.ShinobiError):
+        sa.rebuild_many(ctx, [items[0][3], items[0][3]])
+    with pytest.raises(sa.ShinobiError):
+        sa.rebuild_many(ctx, [items[0][3]], branches=17)
+    ctx.sync()

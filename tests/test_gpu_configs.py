@@ -62,9 +62,10 @@ def test_batched_64_scenes_graph_matches_oracle(ctx, oracle):
         batch.add(it[4], it[5], it[6], it[7])
     batch.commit()
     ctx.sync()
+    # the rebuilds on 16 concurrent branches (sg_rebuild_many), as the
+    # batched measurement runs them
     with sa.Graph.capture(ctx) as graph:
-        for it in items:
-            it[4].rebuild()
+        batch.rebuild(branches=16)
         batch.launch()
     kinds = set()
     expect = []
