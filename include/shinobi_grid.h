@@ -366,8 +366,21 @@ sg_status sg_batch_clear(sg_batch *batch);
 /* append one item: device SoA rays, n rays, device outputs (prim, t) */
 sg_status sg_batch_add(sg_batch *batch, sg_grid *grid, const sg_rays *rays, uint32_t n, uint32_t *d_prim,
                        float *d_t);
-/* upload the item table (stream-ordered; the host side may be changed after) */
+/* upload the item table (stream-ordered; the host side may be changed after)
+ * and the rebuild plan of the items' grids (sg_batch_rebuild) */
 sg_status sg_batch_commit(sg_batch *batch, void *stream);
+/* Rebuild every distinct grid of the committed items (sg_rebuild of each,
+ * from the scene it was last built or rebuilt from and at its resolution,
+ * as of sg_batch_commit).  The uniform grids are rebuilt together, one
+ * launch per build phase for all of them: each launch covers the blocks of
+ * every grid's own launch of that phase, so a few dozen launches replace a
+ * dozen per grid.  Two-level grids are rebuilt on up to branches - 1
+ * concurrent streams beside them, as sg_rebuild_many (branches 0 = the
+ * maximum).  Every grid ends equal to its sg_rebuild.  Enqueues kernels only
+ * and can be captured.  No counterpart in the reference, whose builders run
+ * one scene at a time (FastBuildKernels.h:31-313, called from
+ * FastSimpleGridBuilder.h:43-486). */
+sg_status sg_batch_rebuild(sg_batch *batch, uint32_t branches, void *stream);
 /* trace every committed item, closest hit (shadow = 0) or any hit (1).
  * Enqueues kernels only, so it can be captured with sg_graph_begin/end.
  * Each launch also records, per item, the status bits of the item's grid

@@ -582,25 +582,24 @@ class Batch:
         check(ctx._l.sg_batch_create(ctx.h, max_items, C.byref(self.h)), "sg_batch_create")
         ctx._own(self)
         self._keep: list = []
-        self._grids: list = []
 
     def clear(self) -> None:
         check(self.ctx._l.sg_batch_clear(self.h), "sg_batch_clear")
         self._keep = []
-        self._grids = []
 
     def add(self, grid: "Grid", rays: RayBatch, prim: DeviceBuffer, t: DeviceBuffer, n: int | None = None) -> None:
         r = rays.struct()
         check(self.ctx._l.sg_batch_add(self.h, grid.h, C.byref(r), rays.n if n is None else n, prim.ptr, t.ptr),
               "sg_batch_add")
         self._keep += [grid, rays, prim, t]
-        if all(g is not grid for g in self._grids):
-            self._grids.append(grid)
 
     def rebuild(self, branches: int = 0, stream=None) -> None:
-        """Rebuild every grid of the batch from its scene's current vertices
-        on concurrent branches (sg_rebuild_many)."""
-        rebuild_many(self.ctx, self._grids, branches, stream)
+        """Rebuild every grid of the committed items from its scene's
+        current vertices (sg_batch_rebuild): the uniform grids with one
+        launch per build phase for all of them, the two-level grids on
+        concurrent branches."""
+        check(self.ctx._l.sg_batch_rebuild(self.h, int(branches), self.ctx.stream if stream is None else stream),
+              "sg_batch_rebuild")
 
     def commit(self, stream=None) -> None:
         check(self.ctx._l.sg_batch_commit(self.h, self.ctx.stream if stream is None else stream), "sg_batch_commit")

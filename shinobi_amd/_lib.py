@@ -177,6 +177,7 @@ SIGNATURES = {
     "sg_event_record": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "sg_rebuild_many": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_uint32,
                                   C.c_void_p]),
+    "sg_batch_rebuild": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "sg_stream_wait_event": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "sg_event_elapsed_ms": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     "sg_batch_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),

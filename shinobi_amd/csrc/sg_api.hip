@@ -308,14 +308,9 @@ static sg_status grid_alloc(sg_grid* g) {
 // init+bounds, [2] top emission, [3] top sort, [4] top ranges, then for the
 // two-level grid [5] leaf cells, [6] leaf emission, [7] leaf sort, [8] leaf
 // ranges.
-static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st, void* const* marks = nullptr,
-                          int n_marks = 0) {
-    auto mark = [&](int i) {
-        if (marks && i < n_marks && marks[i]) (void)hipEventRecord((hipEvent_t)marks[i], st);
-    };
+// the build arguments of a grid's (re)build
+static BuildArgs build_args(const sg_grid* g, bool fixed_res) {
     const bool two = g->kind == SG_GRID_TWO_LEVEL;
-    const TriSoA t = scene_soa(g->scene);
-    const uint32_t N = g->n_tris;
     BuildArgs a;
     a.lambda = g->p.lambda;
     a.top_divisor = two ? g->p.top_divisor : 1;
@@ -327,7 +322,12 @@ static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st, void* cons
     a.cap_C2 = g->cap_C2;
     a.cap_P2 = g->cap_P2;
     a.keep_debug = g->p.keep_debug ? 1u : 0u;
+    return a;
+}
 
+// what k_init zeroes before a build of the grid
+static ZeroSegs zero_segs(const sg_grid* g) {
+    const bool two = g->kind == SG_GRID_TWO_LEVEL;
     ZeroSegs z;
     z.count = 0;
     auto seg = [&](void* p, size_t bytes) {
@@ -345,6 +345,49 @@ static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st, void* cons
     }
     // per-pair leaf counts are only written below P1_valid
     seg(g->dbg_pair, (size_t)g->cap_P1 * 4);
+    return z;
+}
+
+}  // extern "C"
+
+bool uni_job_of(sg_grid* g, UniJob* j) {
+    if (!g || !j || g->kind != SG_GRID_UNIFORM || !g->scene || !g->pool) return false;
+    const uint32_t N = g->n_tris;
+    j->t = scene_soa(g->scene);
+    j->n = N;
+    j->h = g->hdr;
+    j->a = build_args(g, true);
+    j->z = zero_segs(g);
+    j->part = g->bpart;
+    j->nb_bounds = (uint32_t)bounds_blocks(N);
+    j->sums = g->sums_top;
+    j->dbg = g->dbg_tri;
+    for (int b = 0; b < 2; ++b) {
+        j->k[b] = g->k1[b];
+        j->v[b] = g->v1[b];
+    }
+    j->sort = g->sort1;
+    j->cap_P = g->cap_P1;
+    j->T = (uint32_t)((g->cap_P1 + 4095u) / 4096u);
+    j->passes = g->passes1;
+    j->rb = g->rb;
+    j->ranges = g->ranges1;
+    j->refs = g->refs;
+    return true;
+}
+
+extern "C" {
+
+static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st, void* const* marks = nullptr,
+                          int n_marks = 0) {
+    auto mark = [&](int i) {
+        if (marks && i < n_marks && marks[i]) (void)hipEventRecord((hipEvent_t)marks[i], st);
+    };
+    const bool two = g->kind == SG_GRID_TWO_LEVEL;
+    const TriSoA t = scene_soa(g->scene);
+    const uint32_t N = g->n_tris;
+    const BuildArgs a = build_args(g, fixed_res);
+    const ZeroSegs z = zero_segs(g);
     mark(0);
     launch_init(g->hdr, N, z, st);
     launch_bounds(t, N, g->bpart, st);
@@ -525,42 +568,51 @@ sg_status sg_rebuild_free_res(sg_ctx* ctx, sg_scene* scene, sg_grid* g, void* st
 // work alone.  Weighs the branch assignment of sg_rebuild_many.
 static constexpr uint64_t kRebuildLaunchTris = 50000;
 
-sg_status sg_rebuild_many(sg_ctx* ctx, uint32_t n, sg_scene* const* scenes, sg_grid* const* grids, uint32_t branches,
-                          void* stream) {
+}  // extern "C"
+
+sg_status rebuild_many_with(sg_ctx* ctx, uint32_t n, sg_scene* const* scenes, sg_grid* const* grids,
+                            uint32_t branches, hipStream_t st, const ForkTask* extra) {
     if (!ctx || branches > SG_REBUILD_BRANCHES_MAX) return SG_ERR_INVALID_ARG;
-    if (n == 0) return SG_OK;
-    if (!scenes || !grids) return SG_ERR_INVALID_ARG;
+    if (n && (!scenes || !grids)) return SG_ERR_INVALID_ARG;
     for (uint32_t i = 0; i < n; ++i) {
         if (!scenes[i] || !grids[i] || scenes[i]->n != grids[i]->n_tris) return SG_ERR_INVALID_ARG;
         for (uint32_t j = 0; j < i; ++j)
             if (grids[j] == grids[i]) return SG_ERR_INVALID_ARG;
     }
-    const uint32_t K = std::min(branches ? branches : (uint32_t)SG_REBUILD_BRANCHES_MAX, n);
+    const uint32_t tasks = n + (extra ? 1u : 0u);
+    if (tasks == 0) return SG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    const uint32_t K = std::min(branches ? branches : (uint32_t)SG_REBUILD_BRANCHES_MAX, tasks);
     if (K == 1) {
+        if (extra) {
+            const sg_status r = extra->run(extra->arg, st);
+            if (r != SG_OK) return r;
+        }
         for (uint32_t i = 0; i < n; ++i) {
-            const sg_status r = rebuild_impl(ctx, scenes[i], grids[i], stream, true);
+            const sg_status r = rebuild_impl(ctx, scenes[i], grids[i], st, true);
             if (r != SG_OK) return r;
         }
         return SG_OK;
     }
-    // largest first, each to the branch with the least work so far
+    // largest first, each to the branch with the least work so far (the
+    // extra task, when there is one, starts branch 0)
     std::vector<uint32_t> order(n);
     for (uint32_t i = 0; i < n; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t a, uint32_t b) { return grids[a]->n_tris > grids[b]->n_tris; });
     std::vector<uint64_t> load(K, 0);
+    if (extra) load[0] = extra->weight_tris + kRebuildLaunchTris;
     std::vector<uint32_t> branch(n);
     for (uint32_t i : order) {
         const uint32_t k = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
         branch[i] = k;
         load[k] += grids[i]->n_tris + kRebuildLaunchTris;
     }
-    HIPCHK(hipSetDevice(ctx->device));
     std::lock_guard<std::mutex> lk(ctx->fork_mu);
-    const hipStream_t st = (hipStream_t)stream;
     HIPCHK(hipEventRecord(ctx->fork_start, st));
     for (uint32_t k = 0; k < K; ++k) HIPCHK(hipStreamWaitEvent(ctx->fork_st[k], ctx->fork_start, 0));
     sg_status r = SG_OK;
+    if (extra) r = extra->run(extra->arg, ctx->fork_st[0]);
     for (uint32_t i : order) {
         if (r == SG_OK) r = rebuild_impl(ctx, scenes[i], grids[i], ctx->fork_st[branch[i]], true);
         // the grid's stream (synchronised before its header is read) is the
@@ -575,6 +627,13 @@ sg_status sg_rebuild_many(sg_ctx* ctx, uint32_t n, sg_scene* const* scenes, sg_g
         HIPCHK(hipStreamWaitEvent(st, ctx->fork_join[k], 0));
     }
     return r;
+}
+
+extern "C" {
+
+sg_status sg_rebuild_many(sg_ctx* ctx, uint32_t n, sg_scene* const* scenes, sg_grid* const* grids, uint32_t branches,
+                          void* stream) {
+    return rebuild_many_with(ctx, n, scenes, grids, branches, (hipStream_t)stream, nullptr);
 }
 
 sg_status sg_rebuild_phased(sg_ctx* ctx, sg_scene* scene, sg_grid* g, void* stream, void* const* events,

@@ -19,8 +19,12 @@ constexpr uint32_t kStage = 4096;              // top pairs a tile stages in LDS
 
 // --------------------------------------------------------------- init
 // Zero the cell ranges and leaves; reset the header.
-__global__ void k_init(GridHeader* h, uint32_t n_tris, ZeroSegs segs) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+// The kernels of the build are written as bodies that take their block
+// index and block count (bid, nb): the kernels below call them with
+// blockIdx.x and gridDim.x, the multi-grid kernels (k_multi_*) with a
+// grid's share of one launch over many grids.
+SG_D void init_body(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, uint32_t bid, uint32_t nb) {
+    if (bid == 0 && threadIdx.x == 0) {
         for (int i = 0; i < 3; ++i) {
             h->bmin_key[i] = 0xFFFFFFFFu;
             h->bmax_key[i] = 0u;
@@ -32,13 +36,17 @@ __global__ void k_init(GridHeader* h, uint32_t n_tris, ZeroSegs segs) {
         h->leaf_sort = 0;
         h->oversize = 0;
     }
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t stride = (size_t)nb * blockDim.x;
     for (int s = 0; s < segs.count; ++s) {
         uint4* p = reinterpret_cast<uint4*>(segs.ptr[s]);
         const size_t n16 = segs.bytes[s] >> 4;
-        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        for (size_t i = (size_t)bid * blockDim.x + threadIdx.x; i < n16; i += stride)
             p[i] = make_uint4(0, 0, 0, 0);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_init(GridHeader* h, uint32_t n_tris, ZeroSegs segs) {
+    init_body(h, n_tris, segs, blockIdx.x, gridDim.x);
 }
 
 SG_D void tri_load(const TriSoA& t, uint32_t i, v3& a, v3& b, v3& c) {
@@ -54,10 +62,10 @@ SG_D void tri_load(const TriSoA& t, uint32_t i, v3& a, v3& b, v3& c) {
 // when the vertices change (sg_scene_create / sg_scene_update, k_prep_tris),
 // not here.  Each block leaves its six partial keys in `part`; k_params
 // folds them (no same-address atomics).
-__global__ __launch_bounds__(kBlock) void k_bounds(TriSoA t, uint32_t n, uint32_t* __restrict__ part) {
+SG_D void bounds_body(const TriSoA& t, uint32_t n, uint32_t* __restrict__ part, uint32_t bid, uint32_t nb) {
     __shared__ uint32_t s_red[6][kBlock / 64];
     uint32_t k[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (uint32_t i = bid * kBlock + threadIdx.x; i < n; i += nb * kBlock) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const uint32_t k0 = fkey(t.v[a][i]), k1 = fkey(t.v[3 + a][i]), k2 = fkey(t.v[6 + a][i]);
@@ -80,8 +88,12 @@ __global__ __launch_bounds__(kBlock) void k_bounds(TriSoA t, uint32_t n, uint32_
         const int a = threadIdx.x;
         uint32_t r = s_red[a][0];
         for (int w = 1; w < kBlock / 64; ++w) r = a < 3 ? min(r, s_red[a][w]) : max(r, s_red[a][w]);
-        part[6 * blockIdx.x + a] = r;
+        part[6 * bid + a] = r;
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bounds(TriSoA t, uint32_t n, uint32_t* __restrict__ part) {
+    bounds_body(t, n, part, blockIdx.x, gridDim.x);
 }
 
 // --------------------------------------------------------------- params
@@ -127,8 +139,7 @@ SG_D void fold_params(GridHeader* h, const BuildArgs& a, const uint32_t (*s_red)
 // FastSimpleGridBuilder.h:114-117 / SimpleGrid.h:99-107.  One block: first
 // the bounds partials of k_bounds' nb blocks are folded into the header;
 // with derive the block then fills the leaf cell size table.
-__global__ __launch_bounds__(kBlock) void k_params(GridHeader* h, BuildArgs a, const uint32_t* __restrict__ part,
-                                                   uint32_t nb, int derive) {
+SG_D void params_body(GridHeader* h, const BuildArgs& a, const uint32_t* __restrict__ part, uint32_t nb, int derive) {
     __shared__ uint32_t s_red[6][kBlock / 64];
     uint32_t k[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
     for (uint32_t b = threadIdx.x; b < nb; b += kBlock)
@@ -150,6 +161,11 @@ __global__ __launch_bounds__(kBlock) void k_params(GridHeader* h, BuildArgs a, c
         const uint32_t d = i >> 8, c = i & 255u;
         h->leaf_size[d][c] = 1.f / (s_crcp[d] * (float)c);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_params(GridHeader* h, BuildArgs a, const uint32_t* __restrict__ part,
+                                                   uint32_t nb, int derive) {
+    params_body(h, a, part, nb, derive);
 }
 
 // grid parameters as every emission lane needs them (uniform loads)
@@ -263,36 +279,39 @@ SG_D void top_counts(const Params& P, const TriSoA& t, uint32_t n, uint32_t i0, 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_top_count(TriSoA t, uint32_t n, GridHeader* h,
-                                                      uint32_t* __restrict__ sums,
-                                                      uint32_t* __restrict__ dbg_counts) {
+SG_D void top_count_body(const TriSoA& t, uint32_t n, const GridHeader* h, uint32_t* __restrict__ sums,
+                         uint32_t* __restrict__ dbg_counts, uint32_t bid) {
     __shared__ uint32_t s_red[kBlock / 64];
     const Params P = read_params(h);
-    const uint32_t i0 = blockIdx.x * kTile + threadIdx.x * kItems;
+    const uint32_t i0 = bid * kTile + threadIdx.x * kItems;
     uint32_t cnt[kItems];
     top_counts(P, t, n, i0, cnt);
     if (dbg_counts)
         for (int j = 0; j < kItems; ++j)
             if (i0 + j < n) dbg_counts[i0 + j] = cnt[j];
     const uint32_t s = block_sum<kBlock / 64>(cnt[0] + cnt[1] + cnt[2] + cnt[3], s_red);
-    if (threadIdx.x == 0) sums[blockIdx.x] = s;
+    if (threadIdx.x == 0) sums[bid] = s;
 }
 
-__global__ __launch_bounds__(kBlock) void k_top_emit(TriSoA t, uint32_t n, GridHeader* h, BuildArgs a,
-                                                     uint32_t* __restrict__ keys,
-                                                     uint32_t* __restrict__ vals,
-                                                     const uint32_t* __restrict__ sums) {
+__global__ __launch_bounds__(kBlock) void k_top_count(TriSoA t, uint32_t n, GridHeader* h,
+                                                      uint32_t* __restrict__ sums,
+                                                      uint32_t* __restrict__ dbg_counts) {
+    top_count_body(t, n, h, sums, dbg_counts, blockIdx.x);
+}
+
+SG_D void top_emit_body(const TriSoA& t, uint32_t n, GridHeader* h, const BuildArgs& a, uint32_t* __restrict__ keys,
+                        uint32_t* __restrict__ vals, const uint32_t* __restrict__ sums, uint32_t bid, uint32_t nb) {
     __shared__ uint32_t s_scan[kBlock / 64];
     __shared__ uint32_t s_k[kStage], s_v[kStage];
     const Params P = read_params(h);
-    const uint32_t prefix = block_prefix_of<kBlock / 64>(sums, blockIdx.x, s_scan);
-    const uint32_t i0 = blockIdx.x * kTile + threadIdx.x * kItems;
+    const uint32_t prefix = block_prefix_of<kBlock / 64>(sums, bid, s_scan);
+    const uint32_t i0 = bid * kTile + threadIdx.x * kItems;
     uint32_t cnt[kItems];
     top_counts(P, t, n, i0, cnt);
     const uint32_t sum = cnt[0] + cnt[1] + cnt[2] + cnt[3];
     uint32_t agg;
     const uint32_t local = block_excl_scan<kBlock / 64>(sum, &agg, s_scan);
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    if (bid == nb - 1 && threadIdx.x == 0) {
         const uint32_t P1 = prefix + agg;
         h->need_P1 = P1;
         h->P1 = P1 <= a.cap_P1 ? P1 : a.cap_P1;
@@ -323,6 +342,13 @@ __global__ __launch_bounds__(kBlock) void k_top_emit(TriSoA t, uint32_t n, GridH
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_top_emit(TriSoA t, uint32_t n, GridHeader* h, BuildArgs a,
+                                                     uint32_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ vals,
+                                                     const uint32_t* __restrict__ sums) {
+    top_emit_body(t, n, h, a, keys, vals, sums, blockIdx.x, gridDim.x);
+}
+
 // ---------------------------------------------------------------- ranges
 // prepareCellRanges / prepareLeafCellRanges
 // (src/Control/Algorithm/FastBuildKernels.h:218-313,
@@ -330,16 +356,15 @@ __global__ __launch_bounds__(kBlock) void k_top_emit(TriSoA t, uint32_t n, GridH
 // on the sorted keys writes {first, last+1} of every key below C; the dummy
 // key's range is not stored.  Also records P_valid = index of the first
 // dummy key.  Cells were zeroed by k_init.
-__global__ __launch_bounds__(kBlock) void k_ranges(const uint32_t* __restrict__ keys, const uint32_t* d_n,
-                                                   uint32_t cap, const uint32_t* d_C,
-                                                   uint32_t* __restrict__ cells, uint32_t* d_pvalid) {
+SG_D void ranges_body(const uint32_t* __restrict__ keys, const uint32_t* d_n, uint32_t cap, const uint32_t* d_C,
+                      uint32_t* __restrict__ cells, uint32_t* d_pvalid, uint32_t bid, uint32_t nb) {
     const uint32_t n = min(*d_n, cap);
     const uint32_t C = *d_C;
     if (n == 0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) *d_pvalid = 0;
+        if (bid == 0 && threadIdx.x == 0) *d_pvalid = 0;
         return;
     }
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (uint32_t i = bid * kBlock + threadIdx.x; i < n; i += nb * kBlock) {
         const uint32_t key = keys[i];
         const uint32_t prev = i > 0 ? keys[i - 1] : 0xFFFFFFFFu;
         const uint32_t next = i + 1 < n ? keys[i + 1] : 0xFFFFFFFFu;
@@ -351,6 +376,12 @@ __global__ __launch_bounds__(kBlock) void k_ranges(const uint32_t* __restrict__ 
             *d_pvalid = i;
         }
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ranges(const uint32_t* __restrict__ keys, const uint32_t* d_n,
+                                                   uint32_t cap, const uint32_t* d_C,
+                                                   uint32_t* __restrict__ cells, uint32_t* d_pvalid) {
+    ranges_body(keys, d_n, cap, d_C, cells, d_pvalid, blockIdx.x, gridDim.x);
 }
 
 // ------------------------------------------------------------ leaf cells
@@ -1080,12 +1111,25 @@ __global__ __launch_bounds__(kBlock) void k_copy_pairs(const uint32_t* __restric
 }
 
 // ---------------------------------------------------------------- launchers
-void launch_init(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, hipStream_t st) {
+static uint32_t init_blocks(const ZeroSegs& segs) {
     size_t total = 0;
     for (int i = 0; i < segs.count; ++i) total += segs.bytes[i] >> 4;
     uint32_t blocks = (uint32_t)((total + kBlock - 1) / kBlock);
-    blocks = blocks < 1 ? 1 : (blocks > 2048 ? 2048 : blocks);
-    hipLaunchKernelGGL(k_init, dim3(blocks), dim3(kBlock), 0, st, h, n_tris, segs);
+    return blocks < 1 ? 1 : (blocks > 2048 ? 2048 : blocks);
+}
+
+static uint32_t ranges_blocks(uint32_t cap) {
+    const uint32_t blocks = (cap + kBlock * 4 - 1) / (kBlock * 4);
+    return blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
+}
+
+static uint32_t refs_blocks(uint32_t cap) {
+    const uint32_t blocks = (cap + kBlock * 4 - 1) / (kBlock * 4);
+    return blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks);
+}
+
+void launch_init(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, hipStream_t st) {
+    hipLaunchKernelGGL(k_init, dim3(init_blocks(segs)), dim3(kBlock), 0, st, h, n_tris, segs);
 }
 
 size_t bounds_blocks(uint32_t n) {
@@ -1113,9 +1157,8 @@ void launch_top_emit(const TriSoA& t, uint32_t n, GridHeader* h, const BuildArgs
 
 void launch_ranges(const uint32_t* keys, const uint32_t* d_n, uint32_t cap, const uint32_t* d_C,
                    uint32_t* cells, uint32_t* d_pvalid, hipStream_t st) {
-    uint32_t blocks = (cap + kBlock * 4 - 1) / (kBlock * 4);
-    blocks = blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
-    hipLaunchKernelGGL(k_ranges, dim3(blocks), dim3(kBlock), 0, st, keys, d_n, cap, d_C, cells, d_pvalid);
+    hipLaunchKernelGGL(k_ranges, dim3(ranges_blocks(cap)), dim3(kBlock), 0, st, keys, d_n, cap, d_C, cells,
+                       d_pvalid);
 }
 
 void launch_leaf_cells(GridHeader* h, const BuildArgs& a, const uint32_t* ranges, uint32_t* cells,
@@ -1164,22 +1207,114 @@ void launch_leaf_sort_fixup(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* 
 // d_gate (optional): nothing to do while *d_gate == 0 (two-level grids: the
 // LDS-sorted leaf tiles wrote their records in k_leaf_emit; only a global
 // leaf sort, GridHeader::leaf_sort > 0, leaves them to this pass)
+SG_D void ref_records_body(const uint32_t* __restrict__ indices, const uint32_t* d_valid, const uint32_t* d_gate,
+                           uint32_t cap, const float4* __restrict__ rec, uint32_t n_tris, float* __restrict__ refs,
+                           uint32_t bid, uint32_t nb) {
+    if (d_gate && *d_gate == 0) return;
+    const uint32_t n = min(*d_valid, cap);
+    for (uint32_t i = bid * kBlock + threadIdx.x; i < n; i += nb * kBlock)
+        write_ref_record(refs, i, rec, indices[i], n_tris);
+}
+
 __global__ __launch_bounds__(kBlock) void k_ref_records(const uint32_t* __restrict__ indices, const uint32_t* d_valid,
                                                         const uint32_t* d_gate, uint32_t cap,
                                                         const float4* __restrict__ rec, uint32_t n_tris,
                                                         float* __restrict__ refs) {
-    if (d_gate && *d_gate == 0) return;
-    const uint32_t n = min(*d_valid, cap);
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-        write_ref_record(refs, i, rec, indices[i], n_tris);
+    ref_records_body(indices, d_valid, d_gate, cap, rec, n_tris, refs, blockIdx.x, gridDim.x);
 }
 
 void launch_ref_records(const uint32_t* indices, const uint32_t* d_valid, const uint32_t* d_gate, uint32_t cap,
                         const float4* rec, uint32_t n_tris, float* refs, hipStream_t st) {
-    uint32_t blocks = (cap + kBlock * 4 - 1) / (kBlock * 4);
-    blocks = blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks);
-    hipLaunchKernelGGL(k_ref_records, dim3(blocks), dim3(kBlock), 0, st, indices, d_valid, d_gate, cap, rec, n_tris,
-                       refs);
+    hipLaunchKernelGGL(k_ref_records, dim3(refs_blocks(cap)), dim3(kBlock), 0, st, indices, d_valid, d_gate, cap,
+                       rec, n_tris, refs);
+}
+
+// ------------------------------------------------ many uniform grids, one launch
+// k_multi_*: one launch of a phase for every job of an entry table; each
+// block runs the phase's body for its job (see UniJob)
+__global__ __launch_bounds__(kBlock) void k_multi_init(const UniJob* __restrict__ jobs,
+                                                       const MultiEntry* __restrict__ e, uint32_t ne) {
+    uint32_t bid, nb;
+    const UniJob& j = jobs[multi_entry(e, ne, blockIdx.x, gridDim.x, bid, nb)];
+    init_body(j.h, j.n, j.z, bid, nb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_multi_bounds(const UniJob* __restrict__ jobs,
+                                                         const MultiEntry* __restrict__ e, uint32_t ne) {
+    uint32_t bid, nb;
+    const UniJob& j = jobs[multi_entry(e, ne, blockIdx.x, gridDim.x, bid, nb)];
+    const TriSoA t = j.t;
+    bounds_body(t, j.n, j.part, bid, nb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_multi_params(const UniJob* __restrict__ jobs,
+                                                         const MultiEntry* __restrict__ e, uint32_t ne) {
+    uint32_t bid, nb;
+    const UniJob& j = jobs[multi_entry(e, ne, blockIdx.x, gridDim.x, bid, nb)];
+    const BuildArgs a = j.a;
+    params_body(j.h, a, j.part, j.nb_bounds, 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_multi_top_count(const UniJob* __restrict__ jobs,
+                                                            const MultiEntry* __restrict__ e, uint32_t ne) {
+    uint32_t bid, nb;
+    const UniJob& j = jobs[multi_entry(e, ne, blockIdx.x, gridDim.x, bid, nb)];
+    const TriSoA t = j.t;
+    top_count_body(t, j.n, j.h, j.sums, j.dbg, bid);
+}
+
+__global__ __launch_bounds__(kBlock) void k_multi_top_emit(const UniJob* __restrict__ jobs,
+                                                           const MultiEntry* __restrict__ e, uint32_t ne) {
+    uint32_t bid, nb;
+    const UniJob& j = jobs[multi_entry(e, ne, blockIdx.x, gridDim.x, bid, nb)];
+    const TriSoA t = j.t;
+    const BuildArgs a = j.a;
+    top_emit_body(t, j.n, j.h, a, j.k[0], j.v[0], j.sums, bid, nb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_multi_ranges(const UniJob* __restrict__ jobs,
+                                                         const MultiEntry* __restrict__ e, uint32_t ne) {
+    uint32_t bid, nb;
+    const UniJob& j = jobs[multi_entry(e, ne, blockIdx.x, gridDim.x, bid, nb)];
+    const int f = j.passes & 1;
+    ranges_body(j.k[f], &j.h->P1, j.cap_P, &j.h->C1, j.ranges, &j.h->P1_valid, bid, nb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_multi_refs(const UniJob* __restrict__ jobs,
+                                                       const MultiEntry* __restrict__ e, uint32_t ne) {
+    uint32_t bid, nb;
+    const UniJob& j = jobs[multi_entry(e, ne, blockIdx.x, gridDim.x, bid, nb)];
+    const int f = j.passes & 1;
+    ref_records_body(j.v[f], &j.h->P1_valid, nullptr, j.cap_P, j.t.rec, j.n, j.refs, bid, nb);
+}
+
+uint32_t multi_blocks(const UniJob& j, MultiPhase ph) {
+    switch (ph) {
+        case MP_INIT: return init_blocks(j.z);
+        case MP_BOUNDS: return j.nb_bounds;
+        case MP_PARAMS: return 1;
+        case MP_TOP_COUNT:
+        case MP_TOP_EMIT: return (j.n + kTile - 1) / kTile;
+        case MP_RANGES: return ranges_blocks(j.cap_P);
+        case MP_REFS: return refs_blocks(j.cap_P);
+        default: return 0;
+    }
+}
+
+void launch_multi(MultiPhase ph, const UniJob* jobs, const MultiEntry* e, uint32_t ne, uint32_t blocks,
+                  hipStream_t st) {
+    if (ne == 0 || blocks == 0) return;
+    const dim3 g(blocks), b(kBlock);
+    switch (ph) {
+        case MP_INIT: hipLaunchKernelGGL(k_multi_init, g, b, 0, st, jobs, e, ne); break;
+        case MP_BOUNDS: hipLaunchKernelGGL(k_multi_bounds, g, b, 0, st, jobs, e, ne); break;
+        case MP_PARAMS: hipLaunchKernelGGL(k_multi_params, g, b, 0, st, jobs, e, ne); break;
+        case MP_TOP_COUNT: hipLaunchKernelGGL(k_multi_top_count, g, b, 0, st, jobs, e, ne); break;
+        case MP_TOP_EMIT: hipLaunchKernelGGL(k_multi_top_emit, g, b, 0, st, jobs, e, ne); break;
+        case MP_RANGES: hipLaunchKernelGGL(k_multi_ranges, g, b, 0, st, jobs, e, ne); break;
+        case MP_REFS: hipLaunchKernelGGL(k_multi_refs, g, b, 0, st, jobs, e, ne); break;
+        default: break;
+    }
 }
 
 // number of per-tile sums the emission kernels need for n items

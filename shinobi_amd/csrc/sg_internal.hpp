@@ -79,6 +79,20 @@ struct sg_grid {
     hipStream_t stream = nullptr;
 };
 
+// ---------------------------------------------- shared by the C ABI units
+// a grid's fixed-resolution rebuild as a job of the multi-grid launches
+// (uniform grids only; false otherwise)
+bool uni_job_of(sg_grid* g, UniJob* j);
+// work enqueued on a stream by a caller of rebuild_many_with
+struct ForkTask {
+    sg_status (*run)(void* arg, hipStream_t st);
+    void* arg;
+    uint64_t weight_tris;   // its work, in triangles of rebuild
+};
+// sg_rebuild_many, with `extra` (optional) enqueued first on branch 0
+sg_status rebuild_many_with(sg_ctx* ctx, uint32_t n, sg_scene* const* scenes, sg_grid* const* grids,
+                            uint32_t branches, hipStream_t st, const ForkTask* extra);
+
 #define HIPCHK(x)                                   \
     do {                                            \
         hipError_t e_ = (x);                        \

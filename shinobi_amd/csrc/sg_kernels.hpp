@@ -142,6 +142,62 @@ int sort_passes(int key_bits, int radix_bits);
 void launch_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, const uint32_t* d_n,
                        uint32_t cap, int key_bits, int radix_bits, uint32_t* scratch, hipStream_t st);
 
+// ------------------------------------------- many uniform grids per launch
+// The rebuild of a set of uniform grids (fixed resolutions) with one launch
+// per phase for the whole set: each launch covers the blocks every grid's
+// own launch would have, numbered one grid after the other.  Entry i of a
+// launch's table names a grid (job) and its first block b0; a block finds
+// its entry by binary search and runs the single-grid kernel's body with
+// its block index and count within the grid, so every grid computes exactly
+// what its own launches compute.
+struct UniJob {
+    TriSoA t;
+    uint32_t n;               // triangles
+    GridHeader* h;
+    BuildArgs a;              // as the grid's rebuild (fixed resolution)
+    ZeroSegs z;               // what the grid's k_init zeroes
+    uint32_t* part;           // bounds partials
+    uint32_t nb_bounds;       // bounds_blocks(n): blocks of the bounds pass
+    uint32_t* sums;           // top emission tile sums
+    uint32_t* dbg;            // per-triangle counts (keep_debug) or null
+    uint32_t* k[2];           // sort buffers: pass p reads k[p & 1], writes k[(p + 1) & 1]
+    uint32_t* v[2];
+    uint32_t* sort;           // sort scratch (sort_scratch_words)
+    uint32_t cap_P;           // pair capacity
+    uint32_t T;               // sort tiles of cap_P
+    int passes, rb;           // sort passes and radix width of the grid
+    uint32_t* ranges;         // cell ranges
+    float* refs;              // reference records
+};
+struct MultiEntry {
+    uint32_t job, b0;
+};
+enum MultiPhase { MP_INIT, MP_BOUNDS, MP_PARAMS, MP_TOP_COUNT, MP_TOP_EMIT, MP_RANGES, MP_REFS, MP_COUNT };
+// the job of block b of a launch over `total` blocks (the last entry with
+// b0 <= b), with the block's index and the block count within the job
+SG_D uint32_t multi_entry(const MultiEntry* __restrict__ e, uint32_t ne, uint32_t b, uint32_t total, uint32_t& bid,
+                          uint32_t& nb) {
+    uint32_t lo = 0, hi = ne;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (e[mid].b0 <= b) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t b0 = e[lo].b0, b1 = lo + 1 < ne ? e[lo + 1].b0 : total;
+    bid = b - b0;
+    nb = b1 - b0;
+    return e[lo].job;
+}
+enum MultiSortStep { MS_UP, MS_SCAN, MS_DOWN };
+// blocks a grid's own launch of the phase uses (0: no launch)
+uint32_t multi_blocks(const UniJob& j, MultiPhase ph);
+uint32_t multi_sort_blocks(const UniJob& j, MultiSortStep step);
+void launch_multi(MultiPhase ph, const UniJob* jobs, const MultiEntry* e, uint32_t ne, uint32_t blocks,
+                  hipStream_t st);
+// pass `pass` of every listed job, all of radix width rb
+void launch_multi_sort(MultiSortStep step, int rb, int pass, const UniJob* jobs, const MultiEntry* e, uint32_t ne,
+                       uint32_t blocks, hipStream_t st);
+
 // scan of n u32 values (reduce-then-scan); scratch: scan_scratch_bytes(n)
 size_t scan_scratch_bytes(uint32_t n);
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, bool inclusive, void* scratch,

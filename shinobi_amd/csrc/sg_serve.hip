@@ -33,17 +33,103 @@ uint32_t chunks_of(uint32_t n) { return (n + 63u) / 64u; }
 }  // namespace
 
 // ================================================================ batching
+// The rebuild plan of a batch's uniform grids: the launches of the
+// multi-grid rebuild (launch_multi / launch_multi_sort) in enqueue order,
+// each over its entries [e0, e0 + ne) of the entry table.
+struct PlanLaunch {
+    bool sort;
+    int phase;        // MultiPhase, or MultiSortStep for a sort launch
+    int rb, pass;     // sort launches
+    uint32_t e0, ne, blocks;
+};
+
+// entries of one job in the plan at most: the seven phases plus three
+// launches per sort pass (keys of at most 32 bits: 8 passes of 4 bits)
+constexpr uint32_t kPlanEntriesPerJob = MP_COUNT + 3 * 8;
+
 struct sg_batch {
     sg_ctx* ctx = nullptr;
     uint32_t cap = 0;
     std::vector<TraceItem> items;
+    std::vector<sg_grid*> grids;   // the items' distinct grids, in sg_batch_add order
     uint32_t next_chunk = 0;
     TraceItem* pinned = nullptr;   // staging of the table for the upload
     TraceItem* dev = nullptr;      // device table
     uint32_t* dev_status = nullptr;// per item: the grid's status bits of the last launch
     hipEvent_t uploaded = nullptr; // the last upload out of `pinned`
     ItemSplit dev_split;           // layout of the device table
+    // rebuild plan (sg_batch_commit): uniform grids as jobs of the
+    // multi-grid launches, two-level grids rebuilt on branches
+    UniJob* pinned_jobs = nullptr;
+    MultiEntry* pinned_entries = nullptr;
+    UniJob* dev_jobs = nullptr;
+    MultiEntry* dev_entries = nullptr;
+    std::vector<PlanLaunch> plan;
+    std::vector<sg_grid*> plan_uniform, plan_two;
+    uint64_t plan_tris = 0;
 };
+
+// Lays out the rebuild plan of the batch's grids in the pinned staging
+// buffers; returns the number of entries.  The launch order is the one of
+// a single uniform rebuild (init, bounds, params, top count / emit, the
+// sort passes, ranges, reference records); the sort passes go by radix
+// width, each pass over the grids that sort with that width and have that
+// many passes.
+static uint32_t plan_rebuild(sg_batch* b) {
+    b->plan.clear();
+    b->plan_uniform.clear();
+    b->plan_two.clear();
+    b->plan_tris = 0;
+    for (sg_grid* g : b->grids) {
+        if (uni_job_of(g, &b->pinned_jobs[b->plan_uniform.size()])) {
+            b->plan_uniform.push_back(g);
+            b->plan_tris += g->n_tris;
+        } else {
+            b->plan_two.push_back(g);
+        }
+    }
+    const UniJob* jobs = b->pinned_jobs;
+    const uint32_t nj = (uint32_t)b->plan_uniform.size();
+    uint32_t ne = 0;
+    auto add = [&](bool sort, int phase, int rb, int pass, auto&& blocks_of) {
+        PlanLaunch L{sort, phase, rb, pass, ne, 0, 0};
+        for (uint32_t i = 0; i < nj; ++i) {
+            const uint32_t nb = blocks_of(jobs[i]);
+            if (nb == 0) continue;
+            b->pinned_entries[ne++] = MultiEntry{i, L.blocks};
+            L.blocks += nb;
+        }
+        L.ne = ne - L.e0;
+        if (L.ne) b->plan.push_back(L);
+    };
+    for (int ph : {MP_INIT, MP_BOUNDS, MP_PARAMS, MP_TOP_COUNT, MP_TOP_EMIT})
+        add(false, ph, 0, 0, [&](const UniJob& j) { return multi_blocks(j, (MultiPhase)ph); });
+    for (int rb : {8, 6, 4}) {
+        int passes = 0;   // at most 8: sort_passes of 32-bit keys at 4 bits
+        for (uint32_t i = 0; i < nj; ++i)
+            if (jobs[i].rb == rb) passes = std::max(passes, jobs[i].passes);
+        for (int p = 0; p < passes; ++p)
+            for (int step : {MS_UP, MS_SCAN, MS_DOWN})
+                add(true, step, rb, p, [&](const UniJob& j) {
+                    return j.rb == rb && j.passes > p ? multi_sort_blocks(j, (MultiSortStep)step) : 0u;
+                });
+    }
+    for (int ph : {MP_RANGES, MP_REFS})
+        add(false, ph, 0, 0, [&](const UniJob& j) { return multi_blocks(j, (MultiPhase)ph); });
+    return ne;
+}
+
+static sg_status run_plan(void* arg, hipStream_t st) {
+    const sg_batch* b = static_cast<const sg_batch*>(arg);
+    for (const PlanLaunch& L : b->plan) {
+        if (L.sort)
+            launch_multi_sort((MultiSortStep)L.phase, L.rb, L.pass, b->dev_jobs, b->dev_entries + L.e0, L.ne,
+                              L.blocks, st);
+        else
+            launch_multi((MultiPhase)L.phase, b->dev_jobs, b->dev_entries + L.e0, L.ne, L.blocks, st);
+    }
+    return SG_OK;
+}
 
 extern "C" {
 
@@ -54,13 +140,23 @@ sg_status sg_batch_create(sg_ctx* ctx, uint32_t max_items, sg_batch** out) {
     b->ctx = ctx;
     b->cap = max_items;
     b->items.reserve(max_items);
+    const size_t n_entries = (size_t)kPlanEntriesPerJob * max_items;
     if (hipHostMalloc((void**)&b->pinned, sizeof(TraceItem) * max_items, hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void**)&b->dev, sizeof(TraceItem) * max_items) != hipSuccess ||
         hipMalloc((void**)&b->dev_status, sizeof(uint32_t) * max_items) != hipSuccess ||
+        hipHostMalloc((void**)&b->pinned_jobs, sizeof(UniJob) * max_items, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&b->pinned_entries, sizeof(MultiEntry) * n_entries, hipHostMallocDefault) !=
+            hipSuccess ||
+        hipMalloc((void**)&b->dev_jobs, sizeof(UniJob) * max_items) != hipSuccess ||
+        hipMalloc((void**)&b->dev_entries, sizeof(MultiEntry) * n_entries) != hipSuccess ||
         hipEventCreate(&b->uploaded) != hipSuccess) {
         if (b->pinned) hipHostFree(b->pinned);
         if (b->dev) hipFree(b->dev);
         if (b->dev_status) hipFree(b->dev_status);
+        if (b->pinned_jobs) hipHostFree(b->pinned_jobs);
+        if (b->pinned_entries) hipHostFree(b->pinned_entries);
+        if (b->dev_jobs) hipFree(b->dev_jobs);
+        if (b->dev_entries) hipFree(b->dev_entries);
         delete b;
         return SG_ERR_OUT_OF_MEMORY;
     }
@@ -76,6 +172,10 @@ sg_status sg_batch_destroy(sg_batch* b) {
     hipHostFree(b->pinned);
     hipFree(b->dev);
     hipFree(b->dev_status);
+    hipHostFree(b->pinned_jobs);
+    hipHostFree(b->pinned_entries);
+    hipFree(b->dev_jobs);
+    hipFree(b->dev_entries);
     delete b;
     return SG_OK;
 }
@@ -83,6 +183,7 @@ sg_status sg_batch_destroy(sg_batch* b) {
 sg_status sg_batch_clear(sg_batch* b) {
     if (!b) return SG_ERR_INVALID_ARG;
     b->items.clear();
+    b->grids.clear();
     b->next_chunk = 0;
     return SG_OK;
 }
@@ -103,6 +204,7 @@ sg_status sg_batch_add(sg_batch* b, sg_grid* g, const sg_rays* rays, uint32_t n,
     it.chunk0 = b->next_chunk;
     b->next_chunk += chunks_of(n);
     b->items.push_back(it);
+    if (std::find(b->grids.begin(), b->grids.end(), g) == b->grids.end()) b->grids.push_back(g);
     return SG_OK;
 }
 
@@ -113,6 +215,13 @@ sg_status sg_batch_commit(sg_batch* b, void* stream) {
     const size_t bytes = sizeof(TraceItem) * b->items.size();
     b->dev_split = split_items(b->items.data(), (uint32_t)b->items.size(), b->pinned);
     if (bytes) HIPCHK(hipMemcpyAsync(b->dev, b->pinned, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    const uint32_t n_entries = plan_rebuild(b);
+    if (!b->plan_uniform.empty()) {
+        HIPCHK(hipMemcpyAsync(b->dev_jobs, b->pinned_jobs, sizeof(UniJob) * b->plan_uniform.size(),
+                              hipMemcpyHostToDevice, (hipStream_t)stream));
+        HIPCHK(hipMemcpyAsync(b->dev_entries, b->pinned_entries, sizeof(MultiEntry) * n_entries,
+                              hipMemcpyHostToDevice, (hipStream_t)stream));
+    }
     // items without rays are never traced: their status reads 0
     if (!b->items.empty())
         HIPCHK(hipMemsetAsync(b->dev_status, 0, sizeof(uint32_t) * b->items.size(), (hipStream_t)stream));
@@ -129,6 +238,21 @@ sg_status sg_batch_get_status(sg_batch* b, void* stream, sg_status* out, uint32_
                           (hipStream_t)stream));
     HIPCHK(hipStreamSynchronize((hipStream_t)stream));
     for (uint32_t i = 0; i < n; ++i) out[i] = untraversable(bits[i]) ? status_from_bits(bits[i]) : SG_OK;
+    return SG_OK;
+}
+
+sg_status sg_batch_rebuild(sg_batch* b, uint32_t branches, void* stream) {
+    if (!b || branches > SG_REBUILD_BRANCHES_MAX) return SG_ERR_INVALID_ARG;
+    std::vector<sg_scene*> scenes;
+    for (sg_grid* g : b->plan_two) scenes.push_back(g->scene);
+    clear_last_error();
+    const ForkTask task{run_plan, b, b->plan_tris};
+    const sg_status r = rebuild_many_with(b->ctx, (uint32_t)b->plan_two.size(), scenes.data(), b->plan_two.data(),
+                                          branches, (hipStream_t)stream, b->plan.empty() ? nullptr : &task);
+    // as sg_rebuild_many: the caller's stream orders the grids' later work
+    for (sg_grid* g : b->plan_uniform) g->stream = (hipStream_t)stream;
+    if (r != SG_OK) return r;
+    HIPCHK(hipGetLastError());
     return SG_OK;
 }
 

@@ -39,14 +39,16 @@ size_t sort_scratch_words(uint32_t cap, int key_bits, int radix_bits) {
 // per-tile digit counts of one pass -> counts[d * T + tile].  Each thread
 // counts 16 consecutive keys and adds runs of equal digits with one LDS
 // atomic, so runs of equal keys (cell-ordered emission) do not serialise.
+// (the kernels are bodies taking the tile / digit index `bid`, called by
+// the single-sort kernels with blockIdx.x and by the multi-grid kernels
+// with the block's index within its grid)
 template <int RB>
-__global__ __launch_bounds__(kBlock) void k_sort_up(const uint32_t* __restrict__ keys, const uint32_t* d_n,
-                                                    uint32_t cap, int shift, uint32_t* __restrict__ counts,
-                                                    uint32_t T) {
+SG_D void sort_up_body(const uint32_t* __restrict__ keys, const uint32_t* d_n, uint32_t cap, int shift,
+                       uint32_t* __restrict__ counts, uint32_t T, uint32_t bid) {
     constexpr int R = 1 << RB;
     __shared__ uint32_t s[R];
     const uint32_t n = min(*d_n, cap);
-    const uint32_t base = blockIdx.x * kSortTile;
+    const uint32_t base = bid * kSortTile;
     if (base >= n) return;
     for (int i = threadIdx.x; i < R; i += kBlock) s[i] = 0;
     __syncthreads();
@@ -78,19 +80,25 @@ __global__ __launch_bounds__(kBlock) void k_sort_up(const uint32_t* __restrict__
     }
     if (rc) atomicAdd(&s[rd], rc);
     __syncthreads();
-    for (int d = threadIdx.x; d < R; d += kBlock) counts[(size_t)d * T + blockIdx.x] = s[d];
+    for (int d = threadIdx.x; d < R; d += kBlock) counts[(size_t)d * T + bid] = s[d];
+}
+
+template <int RB>
+__global__ __launch_bounds__(kBlock) void k_sort_up(const uint32_t* __restrict__ keys, const uint32_t* d_n,
+                                                    uint32_t cap, int shift, uint32_t* __restrict__ counts,
+                                                    uint32_t T) {
+    sort_up_body<RB>(keys, d_n, cap, shift, counts, T, blockIdx.x);
 }
 
 // one block per digit d: counts[d][t] -> sum over t' < t (in place), and
 // totals[d] = the digit's count over all tiles
 template <int RB>
-__global__ __launch_bounds__(kBlock) void k_sort_scan(const uint32_t* d_n, uint32_t cap,
-                                                      uint32_t* __restrict__ totals,
-                                                      uint32_t* __restrict__ counts, uint32_t T) {
+SG_D void sort_scan_body(const uint32_t* d_n, uint32_t cap, uint32_t* __restrict__ totals,
+                         uint32_t* __restrict__ counts, uint32_t T, uint32_t bid) {
     __shared__ uint32_t s_scan[kBlock / 64];
     const uint32_t n = min(*d_n, cap);
     const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
-    const uint32_t d = blockIdx.x;
+    const uint32_t d = bid;
     uint32_t run = 0;
     uint32_t* row = counts + (size_t)d * T;
     // chunks of 256*8 tiles: each thread scans 8 consecutive values
@@ -115,12 +123,17 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan(const uint32_t* d_n, uint3
 }
 
 template <int RB>
-__global__ __launch_bounds__(kBlock) void k_sort_down(const uint32_t* __restrict__ kin,
-                                                      const uint32_t* __restrict__ vin,
-                                                      uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                      const uint32_t* d_n, uint32_t cap, int shift,
-                                                      const uint32_t* __restrict__ totals,
-                                                      const uint32_t* __restrict__ offsets, uint32_t T) {
+__global__ __launch_bounds__(kBlock) void k_sort_scan(const uint32_t* d_n, uint32_t cap,
+                                                      uint32_t* __restrict__ totals,
+                                                      uint32_t* __restrict__ counts, uint32_t T) {
+    sort_scan_body<RB>(d_n, cap, totals, counts, T, blockIdx.x);
+}
+
+template <int RB>
+SG_D void sort_down_body(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                         uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, const uint32_t* d_n,
+                         uint32_t cap, int shift, const uint32_t* __restrict__ totals,
+                         const uint32_t* __restrict__ offsets, uint32_t T, uint32_t bid) {
     constexpr int R = 1 << RB;
     __shared__ uint32_t s_cnt[4][R];
     __shared__ uint32_t s_goff[R];
@@ -129,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_down(const uint32_t* __restrict
     __shared__ uint32_t s_v[kSortTile];
     __shared__ uint32_t s_scan[4];
     const uint32_t n = min(*d_n, cap);
-    const uint32_t tile = blockIdx.x;
+    const uint32_t tile = bid;
     const uint32_t base = tile * kSortTile;
     if (base >= n) return;
     for (int i = threadIdx.x; i < 4 * R; i += kBlock) (&s_cnt[0][0])[i] = 0u;
@@ -207,6 +220,45 @@ __global__ __launch_bounds__(kBlock) void k_sort_down(const uint32_t* __restrict
 }
 
 template <int RB>
+__global__ __launch_bounds__(kBlock) void k_sort_down(const uint32_t* __restrict__ kin,
+                                                      const uint32_t* __restrict__ vin,
+                                                      uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                      const uint32_t* d_n, uint32_t cap, int shift,
+                                                      const uint32_t* __restrict__ totals,
+                                                      const uint32_t* __restrict__ offsets, uint32_t T) {
+    sort_down_body<RB>(kin, vin, kout, vout, d_n, cap, shift, totals, offsets, T, blockIdx.x);
+}
+
+// pass `pass` of the sorts of many grids (UniJob): scratch layout as
+// sort_impl (per-pass digit totals, then the counts matrix)
+template <int RB>
+__global__ __launch_bounds__(kBlock) void k_multi_sort_up(const UniJob* __restrict__ jobs,
+                                                          const MultiEntry* __restrict__ e, uint32_t ne, int pass) {
+    uint32_t bid, nb;
+    const UniJob& j = jobs[multi_entry(e, ne, blockIdx.x, gridDim.x, bid, nb)];
+    sort_up_body<RB>(j.k[pass & 1], &j.h->P1, j.cap_P, pass * RB, j.sort + (size_t)j.passes * (1 << RB), j.T, bid);
+}
+
+template <int RB>
+__global__ __launch_bounds__(kBlock) void k_multi_sort_scan(const UniJob* __restrict__ jobs,
+                                                            const MultiEntry* __restrict__ e, uint32_t ne, int pass) {
+    uint32_t bid, nb;
+    const UniJob& j = jobs[multi_entry(e, ne, blockIdx.x, gridDim.x, bid, nb)];
+    sort_scan_body<RB>(&j.h->P1, j.cap_P, j.sort + (size_t)pass * (1 << RB), j.sort + (size_t)j.passes * (1 << RB),
+                       j.T, bid);
+}
+
+template <int RB>
+__global__ __launch_bounds__(kBlock) void k_multi_sort_down(const UniJob* __restrict__ jobs,
+                                                            const MultiEntry* __restrict__ e, uint32_t ne, int pass) {
+    uint32_t bid, nb;
+    const UniJob& j = jobs[multi_entry(e, ne, blockIdx.x, gridDim.x, bid, nb)];
+    const int i = pass & 1, o = i ^ 1;
+    sort_down_body<RB>(j.k[i], j.v[i], j.k[o], j.v[o], &j.h->P1, j.cap_P, pass * RB,
+                       j.sort + (size_t)pass * (1 << RB), j.sort + (size_t)j.passes * (1 << RB), j.T, bid);
+}
+
+template <int RB>
 static void sort_impl(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, const uint32_t* d_n, uint32_t cap,
                       int key_bits, uint32_t* scratch, hipStream_t st) {
     constexpr int R = 1 << RB;
@@ -224,6 +276,29 @@ static void sort_impl(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, co
                            counts, T);
         uint32_t* t = ki; ki = ko; ko = t;
         t = vi; vi = vo; vo = t;
+    }
+}
+
+uint32_t multi_sort_blocks(const UniJob& j, MultiSortStep step) {
+    return step == MS_SCAN ? (1u << j.rb) : j.T;
+}
+
+template <int RB>
+static void multi_sort_impl(MultiSortStep step, int pass, const UniJob* jobs, const MultiEntry* e, uint32_t ne,
+                            uint32_t blocks, hipStream_t st) {
+    const dim3 g(blocks), b(kBlock);
+    if (step == MS_UP) hipLaunchKernelGGL(k_multi_sort_up<RB>, g, b, 0, st, jobs, e, ne, pass);
+    else if (step == MS_SCAN) hipLaunchKernelGGL(k_multi_sort_scan<RB>, g, b, 0, st, jobs, e, ne, pass);
+    else hipLaunchKernelGGL(k_multi_sort_down<RB>, g, b, 0, st, jobs, e, ne, pass);
+}
+
+void launch_multi_sort(MultiSortStep step, int rb, int pass, const UniJob* jobs, const MultiEntry* e, uint32_t ne,
+                       uint32_t blocks, hipStream_t st) {
+    if (ne == 0 || blocks == 0) return;
+    switch (rb) {
+        case 4: multi_sort_impl<4>(step, pass, jobs, e, ne, blocks, st); break;
+        case 6: multi_sort_impl<6>(step, pass, jobs, e, ne, blocks, st); break;
+        default: multi_sort_impl<8>(step, pass, jobs, e, ne, blocks, st); break;
     }
 }
 
