@@ -125,7 +125,8 @@ sg_status sg_device_count(int *out);
 /* Primitive storage: replaces FaceSoup::upload (src/Control/Structure/
  * FaceSoup.h:171-247).  soa9 = v0.x,v0.y,v0.z,v1.x,...,v2.z, n floats each,
  * in host memory (src_on_device = 0) or device memory (1).  The library keeps
- * a device copy and a 64-byte record per triangle (rewritten by every build). */
+ * a device copy and a 64-byte record per triangle, written when vertices are
+ * uploaded (here and by sg_scene_update) and read by builds and traversal. */
 sg_status sg_scene_create(sg_ctx *ctx, const float *const soa9[9], uint32_t n, int src_on_device,
                           void *stream, sg_scene **out);
 /* new vertex positions, same triangle count: replaces FaceSoup::uploadVertices
@@ -133,7 +134,9 @@ sg_status sg_scene_create(sg_ctx *ctx, const float *const soa9[9], uint32_t n, i
 sg_status sg_scene_update(sg_scene *scene, const float *const soa9[9], int src_on_device, void *stream);
 sg_status sg_scene_destroy(sg_scene *scene);
 uint32_t sg_scene_size(const sg_scene *scene);
-/* device pointers of the scene's SoA arrays (9) */
+/* device pointers of the scene's SoA arrays (9), read-only: new vertices go
+ * through sg_scene_update (src_on_device = 1 for device data), which also
+ * rewrites the triangle records */
 sg_status sg_scene_device_soa(const sg_scene *scene, const float *out9[9]);
 
 /* -------------------------------------------------------------- build --- */
