@@ -93,7 +93,7 @@ enum {
     SG_ARRAY_TOP_VALS = 1,     /* P1  sorted top-level pair triangle ids       */
     SG_ARRAY_TOP_RANGES = 2,   /* 2*C1 top-level [begin,end) (uniform: cells)  */
     SG_ARRAY_TOP_CELLS = 3,    /* 2*C1 packed TwoLevelGridCell                 */
-    SG_ARRAY_LEAF_KEYS = 4,    /* P2                                           */
+    SG_ARRAY_LEAF_KEYS = 4,    /* P2  sorted leaf pair keys (keep_debug)       */
     SG_ARRAY_LEAF_VALS = 5,    /* P2                                           */
     SG_ARRAY_LEAVES = 6,       /* 2*C2 leaf [begin,end)                        */
     SG_ARRAY_INDICES = 7,      /* P   reference -> triangle (FaceSoup::indices) */

@@ -407,7 +407,7 @@ static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st, void* cons
     mark(5);
     launch_leaf_emit(t, g->hdr, a, g->k1[f1], g->v1[f1], g->ranges1, g->cells1, g->k2[0], g->v2[0], g->k2[1],
                      g->v2[1], g->refs, g->dbg_pair, g->sums_leaf, g->leaf_info, g->large_path ? &g->large : nullptr,
-                     st);
+                     g->leaves, st);
     mark(6);
     // Only a grid whose build met leaf tiles too large for the LDS sort has
     // the parallel emission of such tiles and needs the global sort after
@@ -418,10 +418,12 @@ static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st, void* cons
         launch_leaf_sort_fixup(g->k2[0], g->v2[0], g->k2[1], g->v2[1], g->hdr, g->cap_P2, g->key_bits2, g->rb,
                                g->sort2, st);
     mark(7);
-    launch_ranges(g->k2[0], &g->hdr->P2, g->cap_P2, &g->hdr->C2, g->leaves, &g->hdr->P2_valid, st);
-    if (g->large_path)
+    // without the global sort the emission wrote the leaf ranges itself
+    if (g->large_path) {
+        launch_ranges(g->k2[0], &g->hdr->P2, g->cap_P2, &g->hdr->C2, g->leaves, &g->hdr->P2_valid, st);
         launch_ref_records(g->v2[0], &g->hdr->P2_valid, &g->hdr->leaf_sort, g->cap_P2, g->scene->rec, N, g->refs,
                            st);
+    }
     mark(8);
 }
 
@@ -721,7 +723,9 @@ sg_status sg_grid_download(sg_grid* g, int which, void* host, size_t bytes) {
             if (!two) return SG_ERR_INVALID_ARG;
             src = g->cells1; n = (size_t)h.C1 * 8; break;
         case SG_ARRAY_LEAF_KEYS:
-            if (!two) return SG_ERR_INVALID_ARG;
+            // kept by keep_debug grids (and grids with the global leaf sort);
+            // others write their leaf ranges without storing the keys
+            if (!two || !(g->p.keep_debug || g->large_path)) return SG_ERR_INVALID_ARG;
             src = g->k2[f2]; n = (size_t)h.P2 * 4; break;
         case SG_ARRAY_LEAF_VALS:
             if (!two) return SG_ERR_INVALID_ARG;

@@ -849,7 +849,8 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* 
                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                          float* __restrict__ refs,
                                                          const uint32_t* __restrict__ sums,
-                                                         const uint32_t* __restrict__ info, uint32_t T) {
+                                                         const uint32_t* __restrict__ info, uint32_t T,
+                                                         uint32_t* __restrict__ leaves, int write_keys) {
     constexpr int NW = kLeafBlock / 64;
     __shared__ uint32_t s_scan[NW];
     __shared__ uint32_t s_a[kLeafStage], s_b[kLeafStage];
@@ -897,20 +898,30 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* 
     for (uint32_t j = threadIdx.x; j < n; j += kLeafBlock) {
         const uint32_t x = src[j];
         const uint32_t v = top_vals[a0 + (x & ((1u << kPayBits) - 1u))];
+        const uint32_t lk = x >> kPayBits;
         uint32_t idx, key;
         if (j < acc_t) {
             idx = q.pre_acc + j;
-            key = kbase + (x >> kPayBits);
+            key = kbase + lk;
         } else {
             idx = rej_base + (j - acc_t);
             key = L.C2;
         }
         if (idx < cap) {
-            keys[idx] = key;
+            if (write_keys) keys[idx] = key;
             vals[idx] = v;
-            // accepted pairs are at their final slots: their reference
-            // records too (the tile's triangles were read while staging)
-            if (j < acc_t) write_ref_record(refs, idx, t.rec, v, L.n_tris);
+            if (j < acc_t) {
+                // accepted pairs are at their final slots: their reference
+                // records too (the tile's triangles were read while staging)
+                write_ref_record(refs, idx, t.rec, v, L.n_tris);
+                // and the ranges of the tile's leaf cells, which no other
+                // tile holds: {first, last + 1} of each key, as k_ranges
+                // writes them from the sorted keys
+                if (leaves) {
+                    if (j == 0 || (src[j - 1] >> kPayBits) != lk) leaves[2 * (size_t)key] = idx;
+                    if (j + 1 == acc_t || (src[j + 1] >> kPayBits) != lk) leaves[2 * (size_t)key + 1] = idx + 1u;
+                }
+            }
         }
     }
 }
@@ -988,7 +999,8 @@ __global__ __launch_bounds__(kBlock) void k_leaf_emit_large(TriSoA t, GridHeader
                                                             const uint32_t* __restrict__ sums,
                                                             const uint32_t* __restrict__ info, uint32_t T,
                                                             int self_sort, uint32_t* __restrict__ tk,
-                                                            uint32_t* __restrict__ tv, float* __restrict__ refs) {
+                                                            uint32_t* __restrict__ tv, float* __restrict__ refs,
+                                                            uint32_t* __restrict__ leaves) {
     __shared__ uint32_t s_scan[kBlock / 64];
     __shared__ uint32_t s_base[256], s_wc[256 * (kBlock / 64)];
     LeafTilePlan q;
@@ -1052,6 +1064,15 @@ __global__ __launch_bounds__(kBlock) void k_leaf_emit_large(TriSoA t, GridHeader
     }
     for (uint32_t i = threadIdx.x; i < n; i += kBlock)
         write_ref_record(refs, q.pre_acc + i, t.rec, (passes & 1) ? v1[i] : v0[i], L.n_tris);
+    if (!leaves) return;
+    // the ranges of the tile's leaf cells from its sorted keys (as k_leaf_emit)
+    __threadfence_block();
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+        const uint32_t key = k0[i];
+        if (i == 0 || k0[i - 1] != key) leaves[2 * (size_t)key] = q.pre_acc + i;
+        if (i + 1 == n || k0[i + 1] != key) leaves[2 * (size_t)key + 1] = q.pre_acc + i + 1u;
+    }
 }
 
 // Oversize tiles with one thread per top pair: pair i of tile t writes its
@@ -1172,15 +1193,19 @@ void launch_leaf_cells(GridHeader* h, const BuildArgs& a, const uint32_t* ranges
 void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const uint32_t* top_keys,
                       const uint32_t* top_vals, const uint32_t* ranges, const uint32_t* cells, uint32_t* keys,
                       uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, float* refs, uint32_t* dbg,
-                      uint32_t* sums, uint32_t* info, const LeafLarge* large, hipStream_t st) {
+                      uint32_t* sums, uint32_t* info, const LeafLarge* large, uint32_t* leaves, hipStream_t st) {
     const uint32_t T = (uint32_t)leaf_tiles(a.cap_P1);
     const uint32_t blocks = (a.cap_P1 + kTile - 1) / kTile;
     if (blocks)
         hipLaunchKernelGGL(k_leaf_count, dim3(blocks), dim3(kBlock), 0, st, t, h, top_keys, top_vals, ranges, cells,
                            sums, info, T, dbg, large ? large->pacc : nullptr, large ? large->prej : nullptr);
     hipLaunchKernelGGL(k_leaf_scan, dim3(1), dim3(1024), 0, st, h, a, sums, T);
+    // with `large`, the global sort and k_ranges need every key; without it
+    // the tiles write the leaf ranges and the keys are kept for keep_debug
+    uint32_t* lv = large ? nullptr : leaves;
+    const int write_keys = (large || a.keep_debug) ? 1 : 0;
     hipLaunchKernelGGL(k_leaf_emit, dim3(T), dim3(kLeafBlock), 0, st, t, h, a, top_keys, top_vals, cells, keys, vals,
-                       refs, sums, info, T);
+                       refs, sums, info, T, lv, write_keys);
     if (large) {
         launch_scan(large->pacc, large->sacc, a.cap_P1, false, large->scan_scratch, st);
         launch_scan(large->prej, large->srej, a.cap_P1, false, large->scan_scratch, st);
@@ -1190,7 +1215,7 @@ void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const 
                            keys, vals, sums, info, T, large->sacc, large->srej);
     } else {
         hipLaunchKernelGGL(k_leaf_emit_large, dim3(T), dim3(kBlock), 0, st, t, h, a, top_keys, top_vals, cells, keys,
-                           vals, sums, info, T, 1, tmp_keys, tmp_vals, refs);
+                           vals, sums, info, T, 1, tmp_keys, tmp_vals, refs, lv);
     }
 }
 

@@ -122,10 +122,13 @@ struct LeafLarge {
 // (using tmp_keys / tmp_vals, buffers of the pairs' size) and no global leaf
 // sort is needed; with `large`, such tiles are emitted in parallel and left
 // to the global sort (launch_leaf_sort_fixup + launch_ref_records).
+// Without `large` the tiles also write the leaf ranges (`leaves`, zeroed by
+// k_init) and the leaf keys only with a.keep_debug; with it, every key is
+// written for the global sort and k_ranges.
 void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const uint32_t* top_keys,
                       const uint32_t* top_vals, const uint32_t* ranges, const uint32_t* cells, uint32_t* keys,
                       uint32_t* vals, uint32_t* tmp_keys, uint32_t* tmp_vals, float* refs, uint32_t* dbg,
-                      uint32_t* sums, uint32_t* info, const LeafLarge* large, hipStream_t st);
+                      uint32_t* sums, uint32_t* info, const LeafLarge* large, uint32_t* leaves, hipStream_t st);
 // global stable sort of the leaf pairs of tiles too large for LDS (no work
 // when there are none); the result stays in (k0, v0)
 void launch_leaf_sort_fixup(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, GridHeader* h, uint32_t cap,

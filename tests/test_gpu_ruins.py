@@ -25,7 +25,7 @@ def test_ruins_twolevel_build_trace_bounce(ctx, oracle, ruins):
     w = ruins
     print(f"ruins: {w.scene.shape[1]} tris, {w.rays.shape[1]} rays", flush=True)
     scene = sa.FaceSoup(ctx, w.scene)
-    b = sa.FastTwoLevelGridBuilder(ctx, **w.build)
+    b = sa.FastTwoLevelGridBuilder(ctx, keep_debug=1, **w.build)   # keep_debug: the leaf keys too
     b.init(scene)
     g = b.build()
     og = oracle.build_twolevel(w.scene, **w.oracle)
