@@ -34,6 +34,14 @@ HIPFLAGS = [
 ]
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-pthread"]
 
+# Per-source additions.  The traversal kernels are scheduled for
+# instruction-level parallelism (LLVM's max-ilp machine scheduler strategy):
+# on the height field the camera and bounce launches run 2 % faster (0.502
+# -> 0.492 ms, 0.679 -> 0.663 ms) with the same registers and bit-identical
+# results (scheduling reorders independent instructions only; contraction
+# stays off).  The build sources gained nothing from it.
+EXTRA_FLAGS = {"sg_trace.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
+
 
 def _hipcc() -> str:
     for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
@@ -62,7 +70,7 @@ def build(verbose: bool = False) -> str:
     for s in HIP_SOURCES:
         o = os.path.join(OBJ, s.replace(".hip", ".o"))
         objs.append(o)
-        jobs.append([hipcc, *HIPFLAGS, "-c", os.path.join(CSRC, s), "-o", o])
+        jobs.append([hipcc, *HIPFLAGS, *EXTRA_FLAGS.get(s, []), "-c", os.path.join(CSRC, s), "-o", o])
     for s in CXX_SOURCES:
         o = os.path.join(OBJ, s.replace(".cpp", ".o"))
         objs.append(o)
