@@ -608,12 +608,15 @@ SG_D void trace_one(const GView& g, const GRays& rays, uint32_t i, gptr<uint32_t
 
 // Occupancy of the trace kernels: the traversal is bound by memory latency
 // and issue, and waves per SIMD measured 3 / 4 / 5 -> 1.56 / 1.80 / 1.99
-// Grays/s on the height field.  The two-level walk fits 6 waves in 80 VGPRs
-// with a 12-byte spill (per-ray constants reloaded where a leaf walk
-// starts), 1-2 % faster than 5 waves without; at 7 waves the spills reach
-// the test path and it runs 15 % slower.  coop_tests is inlined: as a call
-// it held 12 more VGPRs across the call.
-#define SG_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(6)))
+// Grays/s on the height field.  With the default machine scheduler the
+// two-level walk ran best at 6 waves in 80 VGPRs with a 12-byte spill
+// (+1-2 % over 5 waves, 15 % slower at 7); with the max-ilp scheduler this
+// unit is built with, 5 waves without spills run the height field's bounce
+// rays 4 % faster than 6 (0.667 -> 0.641 ms) and the dense-cluster paths
+// 3 % faster, the ruins scene's bounce rays 4 % slower; the camera launch
+// is unchanged (its kernel fits 6 waves either way).  coop_tests is
+// inlined: as a call it held 12 more VGPRs across the call.
+#define SG_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(5)))
 
 // Rays per wave for a batch of n rays: 64, or fewer (down to 8) while the
 // chip (`fill` waves at once) would not be full.  A small batch -- a late
