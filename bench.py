@@ -51,6 +51,33 @@ PRIMARY_TILE = (2, 32)
 PHASES = ["bounds", "top_emit", "top_sort", "top_ranges", "leaf_cells", "leaf_emit", "leaf_sort", "leaf_ranges"]
 
 
+def hbm_copy_rate(ctx, nbytes: int = 1 << 30, reps: int = 5) -> dict:
+    """The part's streaming rate as measured: the HIP runtime's own
+    device-to-device copy of 1 GiB on the context's stream (read and write
+    both counted), median of `reps` after one warm-up.  Reported beside the
+    spec peak the roofline fractions use."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    hip.hipMemcpyAsync.restype = C.c_int
+    import shinobi_amd as sa
+    src, dst = sa.DeviceBuffer(ctx, nbytes), sa.DeviceBuffer(ctx, nbytes)
+    e0, e1 = ctx.event(), ctx.event()
+    ts = []
+    for k in range(reps + 1):
+        e0.record()
+        if hip.hipMemcpyAsync(dst.ptr, src.ptr, nbytes, 3, ctx.stream) != 0:   # 3: device to device
+            raise RuntimeError("hipMemcpyAsync failed")
+        e1.record()
+        ctx.sync()
+        if k:
+            ts.append(e0.elapsed_ms(e1))
+    ms = float(np.median(ts))
+    del src, dst
+    return {"how": "hipMemcpyAsync device to device, 1 GiB, read + write counted", "ms": round(ms, 4),
+            "gbs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+
+
 def build_bytes(N: int, P1: int, C1: int, P2: int, C2: int) -> dict:
     """Algorithmic HBM bytes of each build phase (36 B per triangle, 8 B per
     pair, 8-bit-digit sort convention); the reference's count / scan / emit
@@ -621,8 +648,10 @@ def main() -> int:
                 "l2_hit": launches[dom].get("l2_hit"),
                 "flops_per_launch": launches[dom]["flops"], "launches": launches,
                 "dominant_phase": dominant[0]}
+    copy = hbm_copy_rate(ctx)
     roofline_build = {"bound": "hbm", "achieved": round(build_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": round(build_gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": total_bytes,
+                      "measured_copy_gbs": copy["gbs"], "frac_of_measured_copy": round(build_gbs / copy["gbs"], 4),
                       "phase_ms": phase_ms, "phase_gbs": phase_gbs}
 
     cpu = None
@@ -660,6 +689,7 @@ def main() -> int:
         "wall_s": round(t_wall, 3),
         "roofline": roofline,
         "roofline_build": roofline_build,
+        "hbm_copy": copy,
         "cpu_baseline": cpu,
     }
     if args.calibrate:
