@@ -257,8 +257,23 @@ def test_rebuild_many_graph_matches_oracle(ctx, oracle):
             np.testing.assert_array_equal(p.download(np.uint32, rb.n), po, err_msg=f"replay {rep}")
             np.testing.assert_array_equal(t.download(np.float32, rb.n).view(np.uint32), to.view(np.uint32))
     graph.close()
-    # eager form, and the argument checks
-    sa.rebuild_many(ctx, [it[3] for it in items], branches=1)
+    # sg_rebuild_many itself, eager, on 1 and on 4 branches (the original
+    # vertices back in between): every grid again equals the oracle's
+    for branches in (1, 4):
+        for it in items:
+            it[2].upload_vertices(it[4])
+        sa.rebuild_many(ctx, [it[3] for it in items], branches=2)
+        for it in items:
+            it[2].upload_vertices(it[5])
+        sa.rebuild_many(ctx, [it[3] for it in items], branches=branches)
+        ctx.sync()
+        for (kind, kw, scene, g, a, moved, res), og in zip(items, expect):
+            _assert_top_level(g, og)
+            if kind:
+                _assert_leaf_level(g, og)
+            else:
+                np.testing.assert_array_equal(g.download("indices"), og.indices)
+    # the argument checks
     sa.rebuild_many(ctx, [], branches=4)
     with pytest.raises(sa.ShinobiError):
         sa.rebuild_many(ctx, [items[0][3], items[0][3]])
