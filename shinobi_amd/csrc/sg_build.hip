@@ -855,6 +855,7 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* 
     __shared__ uint32_t s_scan[NW];
     __shared__ uint32_t s_a[kLeafStage], s_b[kLeafStage];
     __shared__ uint32_t s_base[1 << kDigitBits], s_wc[(1 << kDigitBits) * NW];
+    __shared__ uint32_t s_tri[1u << kPayBits];   // triangle of each top pair of the tile
     LeafTilePlan q;
     if (!leaf_tile_plan(a, sums, info, T, blockIdx.x, q) || !q.staged) return;
     const LeafParams L = read_leaf_params(h);
@@ -869,7 +870,11 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* 
         const uint32_t i = c0 + threadIdx.x;
         LeafSetup s;
         uint32_t c = 0;
-        if (i < a1) c = leaf_pair(t, L, h, top_keys[i], top_vals[i], cells, s);
+        if (i < a1) {
+            const uint32_t tri = top_vals[i];
+            s_tri[i - a0] = tri;
+            c = leaf_pair(t, L, h, top_keys[i], tri, cells, s);
+        }
         uint32_t agg;
         uint32_t p = run + block_excl_scan<NW>(c, &agg, s_scan);
         const uint32_t pay = i - a0;
@@ -897,7 +902,7 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* 
     }
     for (uint32_t j = threadIdx.x; j < n; j += kLeafBlock) {
         const uint32_t x = src[j];
-        const uint32_t v = top_vals[a0 + (x & ((1u << kPayBits) - 1u))];
+        const uint32_t v = s_tri[x & ((1u << kPayBits) - 1u)];
         const uint32_t lk = x >> kPayBits;
         uint32_t idx, key;
         if (j < acc_t) {
