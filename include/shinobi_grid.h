@@ -336,6 +336,11 @@ sg_status sg_sort_pairs(sg_ctx *ctx, uint32_t *d_keys, uint32_t *d_vals, uint32_
                         int radix_bits, void *stream);
 
 /* -------------------------------------------------- device plumbing --- */
+/* Memory, copies, streams and events for hosts that have no HIP binding of
+ * their own (the Python mirror, other languages over the C ABI).  The
+ * reference calls the CUDA runtime directly for these (for instance
+ * cudaMalloc in src/Engine/StaticRTEngine.h:131,192); a C++ host may as
+ * well use HIP itself and pass its pointers and streams here. */
 sg_status sg_malloc(sg_ctx *ctx, size_t bytes, void **out);
 sg_status sg_free(sg_ctx *ctx, void *p);
 /* pinned host memory (page-locked): copies to and from it are DMA transfers
@@ -352,7 +357,8 @@ sg_status sg_event_destroy(sg_ctx *ctx, void *ev);
 sg_status sg_event_record(sg_ctx *ctx, void *ev, void *stream);
 sg_status sg_event_elapsed_ms(sg_ctx *ctx, void *ev_start, void *ev_end, float *ms);
 /* work enqueued on `stream` after this call waits for `ev` (a fork / join
- * between streams; under graph capture it becomes a graph dependency) */
+ * between streams; under graph capture it becomes a graph dependency).  No
+ * counterpart in the reference, which runs everything on one stream. */
 sg_status sg_stream_wait_event(sg_ctx *ctx, void *stream, void *ev);
 
 /* ------------------------------------------------------------ batching --- */
