@@ -125,7 +125,7 @@ sg_status sg_device_count(int *out);
 /* Primitive storage: replaces FaceSoup::upload (src/Control/Structure/
  * FaceSoup.h:171-247).  soa9 = v0.x,v0.y,v0.z,v1.x,...,v2.z, n floats each,
  * in host memory (src_on_device = 0) or device memory (1).  The library keeps
- * a device copy and a 64-byte record per triangle, written when vertices are
+ * a device copy and a 36-byte record per triangle, written when vertices are
  * uploaded (here and by sg_scene_update) and read by builds and traversal. */
 sg_status sg_scene_create(sg_ctx *ctx, const float *const soa9[9], uint32_t n, int src_on_device,
                           void *stream, sg_scene **out);

@@ -183,7 +183,7 @@ sg_status sg_scene_create(sg_ctx* ctx, const float* const soa9[9], uint32_t n, i
     s->n = n;
     s->stride = ((size_t)n + 63) & ~(size_t)63;
     if (hipMalloc(&s->soa, (size_t)9 * s->stride * sizeof(float)) != hipSuccess ||
-        hipMalloc(&s->rec, (size_t)4 * n * sizeof(float4)) != hipSuccess) {
+        hipMalloc(&s->rec, (size_t)kTriRecWords * n * sizeof(float)) != hipSuccess) {
         if (s->soa) hipFree(s->soa);
         delete s;
         return SG_ERR_OUT_OF_MEMORY;

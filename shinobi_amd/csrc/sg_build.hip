@@ -1238,7 +1238,7 @@ void launch_leaf_sort_fixup(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* 
 // LDS-sorted leaf tiles wrote their records in k_leaf_emit; only a global
 // leaf sort, GridHeader::leaf_sort > 0, leaves them to this pass)
 SG_D void ref_records_body(const uint32_t* __restrict__ indices, const uint32_t* d_valid, const uint32_t* d_gate,
-                           uint32_t cap, const float4* __restrict__ rec, uint32_t n_tris, float* __restrict__ refs,
+                           uint32_t cap, const float* __restrict__ rec, uint32_t n_tris, float* __restrict__ refs,
                            uint32_t bid, uint32_t nb) {
     if (d_gate && *d_gate == 0) return;
     const uint32_t n = min(*d_valid, cap);
@@ -1248,13 +1248,13 @@ SG_D void ref_records_body(const uint32_t* __restrict__ indices, const uint32_t*
 
 __global__ __launch_bounds__(kBlock) void k_ref_records(const uint32_t* __restrict__ indices, const uint32_t* d_valid,
                                                         const uint32_t* d_gate, uint32_t cap,
-                                                        const float4* __restrict__ rec, uint32_t n_tris,
+                                                        const float* __restrict__ rec, uint32_t n_tris,
                                                         float* __restrict__ refs) {
     ref_records_body(indices, d_valid, d_gate, cap, rec, n_tris, refs, blockIdx.x, gridDim.x);
 }
 
 void launch_ref_records(const uint32_t* indices, const uint32_t* d_valid, const uint32_t* d_gate, uint32_t cap,
-                        const float4* rec, uint32_t n_tris, float* refs, hipStream_t st) {
+                        const float* rec, uint32_t n_tris, float* refs, hipStream_t st) {
     hipLaunchKernelGGL(k_ref_records, dim3(refs_blocks(cap)), dim3(kBlock), 0, st, indices, d_valid, d_gate, cap,
                        rec, n_tris, refs);
 }

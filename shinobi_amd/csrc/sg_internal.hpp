@@ -44,7 +44,7 @@ struct sg_scene {
     uint32_t n;
     size_t stride = 0;        // floats between the SoA arrays (multiple of 64: 256-byte aligned arrays)
     float* soa = nullptr;     // 9*stride floats
-    float4* rec = nullptr;    // 4*n float4 per-triangle records
+    float* rec = nullptr;     // per-triangle records, kTriRecWords floats each
 };
 
 struct sg_grid {

@@ -5,33 +5,39 @@
 
 namespace sg {
 
+// Unaligned 16-byte vector: packed records are read and written as two of
+// these at 4-byte alignment plus one word.
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+
 // nine SoA arrays: v0.xyz, v1.xyz, v2.xyz (device pointers), plus the
-// per-triangle 64-byte records (4 x float4) that k_prep_tris writes whenever
-// the scene's vertices are set (sg_scene_create / sg_scene_update):
-//   {v0.x, v0.y, v0.z, e1.x} {e1.y, e1.z, e2.x, e2.y} {e2.z, v1.x, v1.y, v1.z} {v2.x, v2.y, v2.z, 0}
-// with e1 = v1 - v0, e2 = v2 - v0.  Traversal reads the first 36 bytes, the
-// leaf level of the build gathers v0/v1/v2 from one aligned record instead
-// of nine SoA arrays.
+// per-triangle records that k_prep_tris writes whenever the scene's vertices
+// are set (sg_scene_create / sg_scene_update): {v0, v1, v2}, nine floats
+// packed at a 36-byte stride (kTriRecWords), so the gathers of the leaf level
+// of the build and of the reference-record writes read one record instead of
+// nine SoA arrays.  Edges are not stored: e1 = v1 - v0 and e2 = v2 - v0 are
+// recomputed where needed, with the same binary32 subtraction, so every
+// result is what stored edges gave.  (A 64-byte record with the edges moved
+// 1.8x the bytes per gather.)
+constexpr uint32_t kTriRecWords = 9;
+
 struct TriSoA {
     const float* v[9];
-    float4* rec;
+    float* rec;
 };
 
-SG_D void store_tri_record(float4* __restrict__ rec, uint32_t i, const v3& v0, const v3& v1, const v3& v2) {
-    const v3 e1 = v1 - v0, e2 = v2 - v0;
-    float4* r = rec + 4 * (size_t)i;
-    r[0] = make_float4(v0.x, v0.y, v0.z, e1.x);
-    r[1] = make_float4(e1.y, e1.z, e2.x, e2.y);
-    r[2] = make_float4(e2.z, v1.x, v1.y, v1.z);
-    r[3] = make_float4(v2.x, v2.y, v2.z, 0.f);
+SG_D void store_tri_record(float* __restrict__ rec, uint32_t i, const v3& v0, const v3& v1, const v3& v2) {
+    float* r = rec + kTriRecWords * (size_t)i;
+    *(f4u*)r = f4u{v0.x, v0.y, v0.z, v1.x};
+    *(f4u*)(r + 4) = f4u{v1.y, v1.z, v2.x, v2.y};
+    r[8] = v2.z;
 }
 
-SG_D void load_tri_record(const float4* __restrict__ rec, uint32_t i, v3& v0, v3& v1, v3& v2) {
-    const float4* r = rec + 4 * (size_t)i;
-    const float4 a = r[0], c = r[2], d = r[3];
+SG_D void load_tri_record(const float* __restrict__ rec, uint32_t i, v3& v0, v3& v1, v3& v2) {
+    const float* r = rec + kTriRecWords * (size_t)i;
+    const f4u a = *(const f4u*)r, b = *(const f4u*)(r + 4);
     v0 = mk(a.x, a.y, a.z);
-    v1 = mk(c.y, c.z, c.w);
-    v2 = mk(d.x, d.y, d.z);
+    v1 = mk(a.w, b.x, b.y);
+    v2 = mk(b.z, b.w, r[8]);
 }
 
 // seven SoA arrays: ox, oy, oz, dx, dy, dz, tmax (device pointers)
@@ -55,7 +61,7 @@ struct GridView {
     const uint32_t* cells;     // uniform: 2*C ranges; two-level: 2*C1 packed cells
     const uint32_t* leaves;    // two-level: 2*C2 ranges
     const uint32_t* indices;   // reference -> triangle id
-    const float4* rec;         // per-triangle record (4 x float4, see TriSoA)
+    const float* rec;          // per-triangle record (9 floats, see TriSoA)
     const float* refs;         // per-reference records (9 floats each, see write_ref_record)
     int kind;                  // 0 uniform, 1 two-level
 };
@@ -69,24 +75,22 @@ struct GridView {
 // 4-byte alignment plus one word: 25 % fewer bytes than 48-byte aligned
 // records, which measured faster on gfx950 (more of them stay in the
 // memory-side cache).
-typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
-
-SG_D void write_ref_record(float* __restrict__ refs, uint32_t i, const float4* __restrict__ rec, uint32_t tri,
+SG_D void write_ref_record(float* __restrict__ refs, uint32_t i, const float* __restrict__ rec, uint32_t tri,
                            uint32_t n_tris) {
-    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
-    float c = 0.f;
+    v3 v0 = mk(0.f, 0.f, 0.f), e1 = v0, e2 = v0;
     if (tri < n_tris) {   // always, unless the build overflowed (never traversed then)
-        a = rec[4 * (size_t)tri];
-        b = rec[4 * (size_t)tri + 1];
-        c = rec[4 * (size_t)tri + 2].x;
+        v3 v1, v2;
+        load_tri_record(rec, tri, v0, v1, v2);
+        e1 = v1 - v0;
+        e2 = v2 - v0;
     }
     float* o = refs + 9 * (size_t)i;
-    *(f4u*)o = f4u{a.x, a.y, a.z, a.w};
-    *(f4u*)(o + 4) = f4u{b.x, b.y, b.z, b.w};
-    o[8] = c;
+    *(f4u*)o = f4u{v0.x, v0.y, v0.z, e1.x};
+    *(f4u*)(o + 4) = f4u{e1.y, e1.z, e2.x, e2.y};
+    o[8] = e2.z;
 }
 void launch_ref_records(const uint32_t* indices, const uint32_t* d_valid, const uint32_t* d_gate, uint32_t cap,
-                        const float4* rec, uint32_t n_tris, float* refs, hipStream_t st);
+                        const float* rec, uint32_t n_tris, float* refs, hipStream_t st);
 void launch_init(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, hipStream_t st);
 // per-block bounds partials (6 words per block, bounds_blocks(n) blocks)
 size_t bounds_blocks(uint32_t n);

@@ -916,9 +916,9 @@ SG_D void bounce_ray(const TriSoA& tris, const RaySoA& rays, uint32_t i, uint32_
                      v3& no, v3& nd) {
     const v3 o = mk(rays.a[0][i], rays.a[1][i], rays.a[2][i]);
     const v3 d = mk(rays.a[3][i], rays.a[4][i], rays.a[5][i]);
-    const float4* rq = tris.rec + 4 * (size_t)tri;
-    const float4 q0 = rq[0], q1 = rq[1], q2 = rq[2];
-    v3 ta = mk(q0.w, q1.x, q1.y), tb = mk(q1.z, q1.w, q2.x);   // v1 - v0, v2 - v0
+    v3 v0, v1, v2;
+    load_tri_record(tris.rec, tri, v0, v1, v2);
+    v3 ta = v1 - v0, tb = v2 - v0;
     v3 nn = normalize(cross(ta, tb));
     ta = normalize(ta);
     tb = cross(nn, ta);
