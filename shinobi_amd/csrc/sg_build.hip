@@ -49,12 +49,6 @@ __global__ __launch_bounds__(kBlock) void k_init(GridHeader* h, uint32_t n_tris,
     init_body(h, n_tris, segs, blockIdx.x, gridDim.x);
 }
 
-SG_D void tri_load(const TriSoA& t, uint32_t i, v3& a, v3& b, v3& c) {
-    a = mk(t.v[0][i], t.v[1][i], t.v[2][i]);
-    b = mk(t.v[3][i], t.v[4][i], t.v[5][i]);
-    c = mk(t.v[6][i], t.v[7][i], t.v[8][i]);
-}
-
 // --------------------------------------------------------------- bounds
 // Scene AABB (the host loop of FastTwoLevelGridBuilder.h:75-81) as a device
 // reduction; exact because min/max never round.  Coalesced SoA loads, one
@@ -204,11 +198,10 @@ SG_D uint32_t top_range(const Params& p, v3 mn, v3 mx, uint32_t lo[3], uint32_t 
 }
 
 // buildUnsortedGrid's per-triangle emission, src/Control/Algorithm/FastBuildKernels.h:130-212
-SG_D void top_emit_one(const Params& P, const TriSoA& t, uint32_t i, uint32_t slot, uint32_t* __restrict__ keys,
-                       uint32_t* __restrict__ vals) {
-    v3 v0, v1, v2, mn, mx;
+SG_D void top_emit_one(const Params& P, const v3& v0, const v3& v1, const v3& v2, uint32_t i, uint32_t slot,
+                       uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    v3 mn, mx;
     uint32_t lo[3], hi[3];
-    tri_load(t, i, v0, v1, v2);
     tri_bounds(v0, v1, v2, mn, mx);
     top_range(P, mn, mx, lo, hi);
     if (hi[0] - lo[0] > 0u && hi[1] - lo[1] > 0u && hi[2] - lo[2] > 0u) {
@@ -250,11 +243,14 @@ SG_D void top_emit_one(const Params& P, const TriSoA& t, uint32_t i, uint32_t sl
 // count kernel leaves one sum per 1024-triangle tile, the emit kernel folds
 // the sums of the tiles before it, rescans its own counts and writes every
 // triangle's pairs at its slot.  Four consecutive triangles per lane (one
-// 16-byte load per SoA array).
-SG_D void top_counts(const Params& P, const TriSoA& t, uint32_t n, uint32_t i0, uint32_t cnt[kItems]) {
+// 16-byte load per SoA array); the emit kernel keeps the lane's vertices in
+// registers from the count to the emission (reloading them after the scans
+// fetched the arrays 2.3 times: a wave of blocks touches more than an XCD's
+// L2 holds).
+SG_D void top_counts(const Params& P, const TriSoA& t, uint32_t n, uint32_t i0, uint32_t cnt[kItems],
+                     float f[9][kItems]) {
     cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;
     if (!P.ok || i0 >= n) return;
-    float f[9][kItems];
     if (i0 + kItems <= n) {
 #pragma unroll
         for (int c = 0; c < 9; ++c) {
@@ -285,7 +281,8 @@ SG_D void top_count_body(const TriSoA& t, uint32_t n, const GridHeader* h, uint3
     const Params P = read_params(h);
     const uint32_t i0 = bid * kTile + threadIdx.x * kItems;
     uint32_t cnt[kItems];
-    top_counts(P, t, n, i0, cnt);
+    float f[9][kItems];
+    top_counts(P, t, n, i0, cnt, f);
     if (dbg_counts)
         for (int j = 0; j < kItems; ++j)
             if (i0 + j < n) dbg_counts[i0 + j] = cnt[j];
@@ -307,7 +304,8 @@ SG_D void top_emit_body(const TriSoA& t, uint32_t n, GridHeader* h, const BuildA
     const uint32_t prefix = block_prefix_of<kBlock / 64>(sums, bid, s_scan);
     const uint32_t i0 = bid * kTile + threadIdx.x * kItems;
     uint32_t cnt[kItems];
-    top_counts(P, t, n, i0, cnt);
+    float f[9][kItems];
+    top_counts(P, t, n, i0, cnt, f);
     const uint32_t sum = cnt[0] + cnt[1] + cnt[2] + cnt[3];
     uint32_t agg;
     const uint32_t local = block_excl_scan<kBlock / 64>(sum, &agg, s_scan);
@@ -324,7 +322,9 @@ SG_D void top_emit_body(const TriSoA& t, uint32_t n, GridHeader* h, const BuildA
         uint32_t ls = local;
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
-            if (cnt[j]) top_emit_one(P, t, i0 + j, ls, s_k, s_v);
+            if (cnt[j])
+                top_emit_one(P, mk(f[0][j], f[1][j], f[2][j]), mk(f[3][j], f[4][j], f[5][j]),
+                             mk(f[6][j], f[7][j], f[8][j]), i0 + j, ls, s_k, s_v);
             ls += cnt[j];
         }
         __syncthreads();
@@ -337,7 +337,9 @@ SG_D void top_emit_body(const TriSoA& t, uint32_t n, GridHeader* h, const BuildA
     uint32_t slot = prefix + local;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
-        if (cnt[j]) top_emit_one(P, t, i0 + j, slot, keys, vals);
+        if (cnt[j])
+            top_emit_one(P, mk(f[0][j], f[1][j], f[2][j]), mk(f[3][j], f[4][j], f[5][j]),
+                         mk(f[6][j], f[7][j], f[8][j]), i0 + j, slot, keys, vals);
         slot += cnt[j];
     }
 }
