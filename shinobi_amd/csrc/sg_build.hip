@@ -844,7 +844,7 @@ SG_D bool leaf_tile_plan(const BuildArgs& a, const uint32_t* __restrict__ sums, 
 }
 
 // tiles that fit the LDS sort
-__global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* h, BuildArgs a,
+__global__ __launch_bounds__(kLeafBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_leaf_emit(TriSoA t, GridHeader* h, BuildArgs a,
                                                          const uint32_t* __restrict__ top_keys,
                                                          const uint32_t* __restrict__ top_vals,
                                                          const uint32_t* __restrict__ cells,
@@ -857,7 +857,7 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* 
     __shared__ uint32_t s_scan[NW];
     __shared__ uint32_t s_a[kLeafStage], s_b[kLeafStage];
     __shared__ uint32_t s_base[1 << kDigitBits], s_wc[(1 << kDigitBits) * NW];
-    __shared__ uint32_t s_tri[1u << kPayBits];   // triangle of each top pair of the tile
+    static_assert(kLeafStage >= (1u << kPayBits), "the spare sort buffer holds the tile's triangle ids");
     LeafTilePlan q;
     if (!leaf_tile_plan(a, sums, info, T, blockIdx.x, q) || !q.staged) return;
     const LeafParams L = read_leaf_params(h);
@@ -872,11 +872,7 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* 
         const uint32_t i = c0 + threadIdx.x;
         LeafSetup s;
         uint32_t c = 0;
-        if (i < a1) {
-            const uint32_t tri = top_vals[i];
-            s_tri[i - a0] = tri;
-            c = leaf_pair(t, L, h, top_keys[i], tri, cells, s);
-        }
+        if (i < a1) c = leaf_pair(t, L, h, top_keys[i], top_vals[i], cells, s);
         uint32_t agg;
         uint32_t p = run + block_excl_scan<NW>(c, &agg, s_scan);
         const uint32_t pay = i - a0;
@@ -902,6 +898,13 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf_emit(TriSoA t, GridHeader* 
         src = dst;
         dst = tmp;
     }
+    // the triangle of each top pair of the tile, in the sort buffer the last
+    // pass read (free now): the write-out reads it there instead of gathering
+    // it from the sorted top pairs.  Sharing the buffer keeps the block's LDS
+    // at 35 KB, four blocks per CU.
+    uint32_t* s_tri = dst;
+    for (uint32_t j = threadIdx.x; j < a1 - a0; j += kLeafBlock) s_tri[j] = top_vals[a0 + j];
+    __syncthreads();
     for (uint32_t j = threadIdx.x; j < n; j += kLeafBlock) {
         const uint32_t x = src[j];
         const uint32_t v = s_tri[x & ((1u << kPayBits) - 1u)];
