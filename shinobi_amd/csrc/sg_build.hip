@@ -51,20 +51,40 @@ __global__ __launch_bounds__(kBlock) void k_init(GridHeader* h, uint32_t n_tris,
 
 // --------------------------------------------------------------- bounds
 // Scene AABB (the host loop of FastTwoLevelGridBuilder.h:75-81) as a device
-// reduction; exact because min/max never round.  Coalesced SoA loads, one
-// triangle per lane.  The per-triangle records (TriSoA::rec) are written
-// when the vertices change (sg_scene_create / sg_scene_update, k_prep_tris),
-// not here.  Each block leaves its six partial keys in `part`; k_params
-// folds them (no same-address atomics).
+// reduction; exact because min/max never round.  Coalesced SoA loads, four
+// consecutive triangles per lane (one 16-byte load per SoA array, as the top
+// count and emission read them; the arrays are 16-byte aligned).  The
+// per-triangle records (TriSoA::rec) are written when the vertices change
+// (sg_scene_create / sg_scene_update, k_prep_tris), not here.  Each block
+// leaves its six partial keys in `part`; k_params folds them (no
+// same-address atomics).
 SG_D void bounds_body(const TriSoA& t, uint32_t n, uint32_t* __restrict__ part, uint32_t bid, uint32_t nb) {
     __shared__ uint32_t s_red[6][kBlock / 64];
     uint32_t k[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-    for (uint32_t i = bid * kBlock + threadIdx.x; i < n; i += nb * kBlock) {
+    for (uint32_t i0 = (bid * kBlock + threadIdx.x) * 4u; i0 < n; i0 += nb * kBlock * 4u) {
+        if (i0 + 4u <= n) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const uint32_t k0 = fkey(t.v[a][i]), k1 = fkey(t.v[3 + a][i]), k2 = fkey(t.v[6 + a][i]);
-            k[a] = min(k[a], min(k0, min(k1, k2)));
-            k[3 + a] = max(k[3 + a], max(k0, max(k1, k2)));
+            for (int a = 0; a < 3; ++a) {
+                const float4 p = reinterpret_cast<const float4*>(t.v[a])[i0 / 4];
+                const float4 q = reinterpret_cast<const float4*>(t.v[3 + a])[i0 / 4];
+                const float4 r = reinterpret_cast<const float4*>(t.v[6 + a])[i0 / 4];
+                const uint32_t m0 = min(min(fkey(p.x), fkey(p.y)), min(fkey(p.z), fkey(p.w)));
+                const uint32_t m1 = min(min(fkey(q.x), fkey(q.y)), min(fkey(q.z), fkey(q.w)));
+                const uint32_t m2 = min(min(fkey(r.x), fkey(r.y)), min(fkey(r.z), fkey(r.w)));
+                const uint32_t x0 = max(max(fkey(p.x), fkey(p.y)), max(fkey(p.z), fkey(p.w)));
+                const uint32_t x1 = max(max(fkey(q.x), fkey(q.y)), max(fkey(q.z), fkey(q.w)));
+                const uint32_t x2 = max(max(fkey(r.x), fkey(r.y)), max(fkey(r.z), fkey(r.w)));
+                k[a] = min(k[a], min(m0, min(m1, m2)));
+                k[3 + a] = max(k[3 + a], max(x0, max(x1, x2)));
+            }
+        } else {
+            for (uint32_t i = i0; i < n; ++i)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const uint32_t k0 = fkey(t.v[a][i]), k1 = fkey(t.v[3 + a][i]), k2 = fkey(t.v[6 + a][i]);
+                    k[a] = min(k[a], min(k0, min(k1, k2)));
+                    k[3 + a] = max(k[3 + a], max(k0, max(k1, k2)));
+                }
         }
     }
 #pragma unroll
@@ -136,8 +156,21 @@ SG_D void fold_params(GridHeader* h, const BuildArgs& a, const uint32_t (*s_red)
 SG_D void params_body(GridHeader* h, const BuildArgs& a, const uint32_t* __restrict__ part, uint32_t nb, int derive) {
     __shared__ uint32_t s_red[6][kBlock / 64];
     uint32_t k[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-    for (uint32_t b = threadIdx.x; b < nb; b += kBlock)
-        for (int i = 0; i < 6; ++i) k[i] = i < 3 ? min(k[i], part[6 * b + i]) : max(k[i], part[6 * b + i]);
+    // four partials per thread and step, loaded before any is folded (one
+    // memory round trip per step, not per partial)
+    for (uint32_t b0 = threadIdx.x; b0 < nb; b0 += 4u * kBlock) {
+        uint32_t v[4][6];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t b = b0 + (uint32_t)u * kBlock;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) v[u][i] = b < nb ? part[6 * b + i] : (i < 3 ? 0xFFFFFFFFu : 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int i = 0; i < 6; ++i) k[i] = i < 3 ? min(k[i], v[u][i]) : max(k[i], v[u][i]);
+    }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1)
         for (int i = 0; i < 6; ++i) {
@@ -1164,7 +1197,7 @@ void launch_init(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, hipStream
 }
 
 size_t bounds_blocks(uint32_t n) {
-    const size_t b = ((size_t)n + kBlock * 4 - 1) / (kBlock * 4);   // four 256-triangle chunks per block
+    const size_t b = ((size_t)n + kBlock * 4 - 1) / (kBlock * 4);   // 1024 triangles per block, four per lane
     return b < 1 ? 1 : (b > 4096 ? 4096 : b);
 }
 
