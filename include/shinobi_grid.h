@@ -386,7 +386,9 @@ sg_status sg_batch_commit(sg_batch *batch, void *stream);
  * dozen per grid.  Two-level grids are rebuilt on up to branches - 1
  * concurrent streams beside them, as sg_rebuild_many (branches 0 = the
  * maximum).  Every grid ends equal to its sg_rebuild.  Enqueues kernels only
- * and can be captured.  No counterpart in the reference, whose builders run
+ * and can be captured.  SG_ERR_INVALID_ARG when a grid was rebuilt from
+ * another scene after the commit (commit again first).  No counterpart in
+ * the reference, whose builders run
  * one scene at a time (FastBuildKernels.h:31-313, called from
  * FastSimpleGridBuilder.h:43-486). */
 sg_status sg_batch_rebuild(sg_batch *batch, uint32_t branches, void *stream);

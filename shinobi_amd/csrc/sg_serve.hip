@@ -66,6 +66,10 @@ struct sg_batch {
     MultiEntry* dev_entries = nullptr;
     std::vector<PlanLaunch> plan;
     std::vector<sg_grid*> plan_uniform, plan_two;
+    // each planned grid's scene at the commit: the uniform jobs hold that
+    // scene's device arrays, so a grid rebuilt from another scene since
+    // must be committed again before sg_batch_rebuild
+    std::vector<sg_scene*> plan_uniform_scene, plan_two_scene;
     uint64_t plan_tris = 0;
 };
 
@@ -79,13 +83,17 @@ static uint32_t plan_rebuild(sg_batch* b) {
     b->plan.clear();
     b->plan_uniform.clear();
     b->plan_two.clear();
+    b->plan_uniform_scene.clear();
+    b->plan_two_scene.clear();
     b->plan_tris = 0;
     for (sg_grid* g : b->grids) {
         if (uni_job_of(g, &b->pinned_jobs[b->plan_uniform.size()])) {
             b->plan_uniform.push_back(g);
+            b->plan_uniform_scene.push_back(g->scene);
             b->plan_tris += g->n_tris;
         } else {
             b->plan_two.push_back(g);
+            b->plan_two_scene.push_back(g->scene);
         }
     }
     const UniJob* jobs = b->pinned_jobs;
@@ -185,6 +193,13 @@ sg_status sg_batch_clear(sg_batch* b) {
     b->items.clear();
     b->grids.clear();
     b->next_chunk = 0;
+    // the rebuild plan goes with the items (its grids may be destroyed next)
+    b->plan.clear();
+    b->plan_uniform.clear();
+    b->plan_two.clear();
+    b->plan_uniform_scene.clear();
+    b->plan_two_scene.clear();
+    b->plan_tris = 0;
     return SG_OK;
 }
 
@@ -243,12 +258,15 @@ sg_status sg_batch_get_status(sg_batch* b, void* stream, sg_status* out, uint32_
 
 sg_status sg_batch_rebuild(sg_batch* b, uint32_t branches, void* stream) {
     if (!b || branches > SG_REBUILD_BRANCHES_MAX) return SG_ERR_INVALID_ARG;
-    std::vector<sg_scene*> scenes;
-    for (sg_grid* g : b->plan_two) scenes.push_back(g->scene);
+    for (size_t i = 0; i < b->plan_uniform.size(); ++i)
+        if (b->plan_uniform[i]->scene != b->plan_uniform_scene[i]) return SG_ERR_INVALID_ARG;
+    for (size_t i = 0; i < b->plan_two.size(); ++i)
+        if (b->plan_two[i]->scene != b->plan_two_scene[i]) return SG_ERR_INVALID_ARG;
     clear_last_error();
     const ForkTask task{run_plan, b, b->plan_tris};
-    const sg_status r = rebuild_many_with(b->ctx, (uint32_t)b->plan_two.size(), scenes.data(), b->plan_two.data(),
-                                          branches, (hipStream_t)stream, b->plan.empty() ? nullptr : &task);
+    const sg_status r =
+        rebuild_many_with(b->ctx, (uint32_t)b->plan_two.size(), b->plan_two_scene.data(), b->plan_two.data(),
+                          branches, (hipStream_t)stream, b->plan.empty() ? nullptr : &task);
     // as sg_rebuild_many: the caller's stream orders the grids' later work
     for (sg_grid* g : b->plan_uniform) g->stream = (hipStream_t)stream;
     if (r != SG_OK) return r;

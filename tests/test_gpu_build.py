@@ -280,3 +280,27 @@ def test_rebuild_many_graph_matches_oracle(ctx, oracle):
     with pytest.raises(sa.ShinobiError):
         sa.rebuild_many(ctx, [items[0][3]], branches=17)
     ctx.sync()
+    # a grid rebuilt from another scene (same triangle count) after the
+    # commit: the batch refuses to rebuild until it is committed again, a
+    # cleared batch rebuilds nothing, and after the new commit the grid is
+    # rebuilt from the new scene
+    kind, kw, scene, g, a, moved, res = items[0]
+    assert kind == 0
+    other = sa.gen_soup(91, a.shape[1], 0.01)
+    oscene = sa.FaceSoup(ctx, other)
+    sa.check(ctx._l.sg_rebuild(ctx.h, oscene.h, g.h, ctx.stream), "sg_rebuild")
+    with pytest.raises(sa.ShinobiError):
+        batch.rebuild(branches=4)
+    batch.clear()
+    batch.rebuild(branches=4)
+    for it, (p, t) in zip(items, outs):
+        batch.add(it[3], rb, p, t)
+    batch.commit()
+    oscene.upload_vertices(other * np.float32(0.9))
+    batch.rebuild(branches=4)
+    ctx.sync()
+    og = oracle.build_uniform(other * np.float32(0.9), top_divisor=1, res_override=res, **kw)
+    assert og.status == 0
+    _assert_top_level(g, og)
+    np.testing.assert_array_equal(g.download("indices"), og.indices)
+    batch.clear()
