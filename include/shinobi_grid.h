@@ -372,7 +372,9 @@ typedef struct sg_batch sg_batch;
 sg_status sg_batch_create(sg_ctx *ctx, uint32_t max_items, sg_batch **out);
 sg_status sg_batch_destroy(sg_batch *batch);
 sg_status sg_batch_clear(sg_batch *batch);
-/* append one item: device SoA rays, n rays, device outputs (prim, t) */
+/* append one item: device SoA rays, n rays, device outputs (prim, t).  The
+ * batch keeps the pointers: the grid, rays and outputs must outlive the
+ * item (sg_batch_clear / sg_batch_destroy end it). */
 sg_status sg_batch_add(sg_batch *batch, sg_grid *grid, const sg_rays *rays, uint32_t n, uint32_t *d_prim,
                        float *d_t);
 /* upload the item table (stream-ordered; the host side may be changed after)
