@@ -157,15 +157,26 @@ def counters_of(launch: str) -> dict:
             "waves_per_simd": k.get("waves_per_simd")}
 
 
-def cpu_baseline(w, n_primary: int, seed: int, threads: int, runs: int = 5) -> dict:
+def cpu_baseline(w, n_primary: int, seed: int, threads: int, runs: int = 5, w_small=None) -> dict:
     """The oracle (the restatement of the reference's GPU path, compiled
     -O2 for the host) on this host's cores: two-level build, closest-hit
     trace of the primary rays, one bounce per hit, trace of the bounce rays.
     Median of `runs` runs after one warm-up, with all usable cores and with
-    one core."""
+    one core.  With `w_small` (the 10,000-triangle uniform soup and its
+    256x256 camera rays), also that workload's uniform build and trace."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import py_oracle as O
     O.load()
+
+    def small(nt):
+        t0 = time.perf_counter()
+        og = O.build_uniform(w_small.scene, nthreads=nt, **w_small.oracle)
+        t_build = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        og.trace(w_small.rays, nthreads=nt)
+        t_trace = time.perf_counter() - t0
+        og.close()
+        return t_build, t_trace
 
     def once(nt):
         t0 = time.perf_counter()
@@ -190,12 +201,23 @@ def cpu_baseline(w, n_primary: int, seed: int, threads: int, runs: int = 5) -> d
         rays = n_primary + rows[0][2]
         res[nt] = (round(rays / t_trace / 1e6, 3), round(t_build * 1e3, 1), rows[0][2])
     cpu = host_cpu()
-    return {"value": res[threads][0], "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "build_ms": res[threads][1], "one_core": {"value": res[1][0], "unit": "Mrays/s", "build_ms": res[1][1]},
-            "cpu_model": cpu["model"], "host": cpu, "runs": f"median of {runs} after 1 warm-up",
-            "sample": f"full workload on the host: oracle two-level build + closest-hit trace of {n_primary} "
-                      f"primary and {res[threads][2]} bounce rays (row-major pixel order), {threads} threads "
-                      f"and 1 thread"}
+    out = {"value": res[threads][0], "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "build_ms": res[threads][1], "one_core": {"value": res[1][0], "unit": "Mrays/s", "build_ms": res[1][1]},
+           "cpu_model": cpu["model"], "host": cpu, "runs": f"median of {runs} after 1 warm-up",
+           "sample": f"full workload on the host: oracle two-level build + closest-hit trace of {n_primary} "
+                     f"primary and {res[threads][2]} bounce rays (row-major pixel order), {threads} threads "
+                     f"and 1 thread"}
+    if w_small is not None:
+        n_small = w_small.rays.shape[1]
+        sm = {}
+        for nt in (threads, 1):
+            small(nt)
+            rows = [small(nt) for _ in range(runs)]
+            sm[nt] = {"build_ms": round(float(np.median([r[0] for r in rows])) * 1e3, 3),
+                      "mrays_per_s": round(n_small / float(np.median([r[1] for r in rows])) / 1e6, 3)}
+        out["soup10k"] = {"cores": threads, **sm[threads], "one_core": sm[1],
+                          "sample": f"oracle uniform build + closest-hit trace of {n_small} camera rays"}
+    return out
 
 
 def _log_uniform(rng, lo, hi, n):
@@ -382,6 +404,37 @@ def ruins_bench(sa, W, ctx, reps: int = 10) -> dict:
             "C1": info.C1, "P1": info.P1, "C2": info.C2, "P2": info.P2, "build_ms": round(float(r[0]), 4),
             "trace_primary_ms": round(float(r[1]), 4), "trace_bounce_ms": round(float(r[3]), 4),
             "mrays_per_s": round((n + m) / (r[1] + r[3]) / 1e3, 1)}
+
+
+def soup10k_bench(sa, W, ctx, reps: int = 20) -> dict:
+    """The reference's CPU-path configuration on the GPU: 10,000 triangles
+    (centroids uniform in the unit cube, vertices within 0.01), uniform grid
+    (lambda 5), 256x256 pinhole camera rays from (0.5, 0.5, -1) along +z,
+    60 degrees; rebuild and camera trace (rays generated in the kernel)."""
+    w = W.soup10k()
+    scene = sa.FaceSoup(ctx, w.scene)
+    b = sa.FastSimpleGridBuilder(ctx, **w.build)
+    b.init(scene)
+    g = b.build()
+    info = g.info()
+    n = w.rays.shape[1]
+    cam = sa.pinhole(*w.camera)
+    prim, t = sa.DeviceBuffer(ctx, 4 * n), sa.DeviceBuffer(ctx, 4 * n)
+    ev = [ctx.event() for _ in range(3)]
+    rows = []
+    for rep in range(reps + 2):
+        ev[0].record()
+        g.rebuild()
+        ev[1].record()
+        g.trace_camera(cam, PRIMARY_TILE, prim=prim, t=t)
+        ev[2].record()
+        ctx.sync()
+        if rep >= 2:
+            rows.append([ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2])])
+    r = np.median(np.array(rows), axis=0)
+    return {"n_tris": int(w.scene.shape[1]), "rays": n, "res": list(info.res), "C": info.C1, "P": info.P1,
+            "build_ms": round(float(r[0]), 4), "trace_ms": round(float(r[1]), 4),
+            "mrays_per_s": round(n / r[1] / 1e3, 1)}
 
 
 BATCH_BRANCHES = 16   # concurrent rebuild branches of the batched graph
@@ -657,7 +710,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or host_cpu()["cores"]
-        cpu = cpu_baseline(w, n_rays, w.bounce_seed, threads)
+        cpu = cpu_baseline(w, n_rays, w.bounce_seed, threads, w_small=W.soup10k())
 
     out = {
         "metric": "closest-hit Mrays/s + grid build ms (1 GPU)",
@@ -700,6 +753,7 @@ def main() -> int:
             sa.check(ctx._l.sg_scan_u32(ctx.h, d_in.ptr, d_out.ptr, n_cal, 0, ctx.stream), "sg_scan_u32")
         ctx.sync()
         del d_in, d_out
+    out["soup10k"] = soup10k_bench(sa, W, ctx)
     if not args.no_ruins:
         out["ruins"] = ruins_bench(sa, W, ctx)
     if not args.no_cluster:
