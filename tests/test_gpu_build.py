@@ -99,6 +99,25 @@ def test_twolevel_cluster_bit_exact(ctx, oracle):
     _assert_leaf_level(g, og)
 
 
+def test_twolevel_cluster_full_size_bit_exact(ctx, oracle):
+    """the dense-cluster configuration at full size (500,000 tiny triangles
+    in a sphere of radius 0.02 plus 2,000 large ones in [0, 10]^3): every
+    top- and leaf-level array equals the oracle's, after the build and after
+    a rebuild"""
+    w = W.cluster(res_x=16, res_y=16)
+    assert w.scene.shape[1] == 502_000
+    og = oracle.build_twolevel(w.scene, **w.oracle)
+    assert og.status == 0
+    _, g = _gpu_build(ctx, w.scene, 1, **w.build)
+    _assert_same_params(g.info(), og)
+    _assert_top_level(g, og)
+    _assert_leaf_level(g, og)
+    g.rebuild()
+    _assert_same_params(g.info(), og)
+    _assert_top_level(g, og)
+    _assert_leaf_level(g, og)
+
+
 def test_rebuild_into_an_oversize_tile(ctx, oracle):
     """a grid built without oversize leaf tiles and rebuilt after 5,000 of its
     triangles moved into one top cell: the one-block emission of such tiles
