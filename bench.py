@@ -546,6 +546,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-scene measurement")
     ap.add_argument("--no-cluster", action="store_true", help="skip the dense-cluster path measurement")
     ap.add_argument("--no-ruins", action="store_true", help="skip the reference ruins scene measurement")
+    ap.add_argument("--no-soup10k", action="store_true", help="skip the 10,000-triangle uniform measurement")
     ap.add_argument("--no-rowmajor", action="store_true", help="skip the row-major primary trace measurement")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 64 Mi-element scan (k_scan_up reads 256 MiB, k_scan_down reads and writes "
@@ -753,7 +754,8 @@ def main() -> int:
             sa.check(ctx._l.sg_scan_u32(ctx.h, d_in.ptr, d_out.ptr, n_cal, 0, ctx.stream), "sg_scan_u32")
         ctx.sync()
         del d_in, d_out
-    out["soup10k"] = soup10k_bench(sa, W, ctx)
+    if not args.no_soup10k:
+        out["soup10k"] = soup10k_bench(sa, W, ctx)
     if not args.no_ruins:
         out["ruins"] = ruins_bench(sa, W, ctx)
     if not args.no_cluster:

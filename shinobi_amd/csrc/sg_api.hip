@@ -325,7 +325,7 @@ static BuildArgs build_args(const sg_grid* g, bool fixed_res) {
     return a;
 }
 
-// what k_init zeroes before a build of the grid
+// what the first launch of a build (k_init_bounds, k_multi_init) zeroes
 static ZeroSegs zero_segs(const sg_grid* g) {
     const bool two = g->kind == SG_GRID_TWO_LEVEL;
     ZeroSegs z;
@@ -389,8 +389,7 @@ static void enqueue_build(sg_grid* g, bool fixed_res, hipStream_t st, void* cons
     const BuildArgs a = build_args(g, fixed_res);
     const ZeroSegs z = zero_segs(g);
     mark(0);
-    launch_init(g->hdr, N, z, st);
-    launch_bounds(t, N, g->bpart, st);
+    launch_init_bounds(g->hdr, N, z, t, g->bpart, st);
     launch_params(g->hdr, a, g->bpart, N, true, st);
     mark(1);
     launch_top_emit(t, N, g->hdr, a, g->k1[0], g->v1[0], g->dbg_tri, g->sums_top, st);
@@ -467,8 +466,7 @@ sg_status sg_build(sg_ctx* ctx, sg_scene* scene, const sg_build_params* p, void*
         }
         ZeroSegs z;
         z.count = 0;
-        launch_init(g->hdr, g->n_tris, z, st);
-        launch_bounds(scene_soa(scene), g->n_tris, g->bpart, st);
+        launch_init_bounds(g->hdr, g->n_tris, z, scene_soa(scene), g->bpart, st);
         launch_params(g->hdr, BuildArgs(), g->bpart, g->n_tris, false, st);
         GridHeader h;
         r = read_header(g, &h);

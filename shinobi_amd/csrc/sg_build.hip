@@ -45,9 +45,6 @@ SG_D void init_body(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, uint32
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_init(GridHeader* h, uint32_t n_tris, ZeroSegs segs) {
-    init_body(h, n_tris, segs, blockIdx.x, gridDim.x);
-}
 
 // --------------------------------------------------------------- bounds
 // Scene AABB (the host loop of FastTwoLevelGridBuilder.h:75-81) as a device
@@ -106,8 +103,14 @@ SG_D void bounds_body(const TriSoA& t, uint32_t n, uint32_t* __restrict__ part, 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bounds(TriSoA t, uint32_t n, uint32_t* __restrict__ part) {
-    bounds_body(t, n, part, blockIdx.x, gridDim.x);
+// The first launch of a build: the zeroing of the build's segments and the
+// bounds reduction, which touch disjoint memory, share one launch (their
+// writes and reads then overlap instead of following each other).  Blocks
+// below nb_bounds also reduce the bounds; every block zeroes.
+__global__ __launch_bounds__(kBlock) void k_init_bounds(GridHeader* h, uint32_t n_tris, ZeroSegs segs, TriSoA t,
+                                                        uint32_t* __restrict__ part, uint32_t nb_bounds) {
+    init_body(h, n_tris, segs, blockIdx.x, gridDim.x);
+    if (blockIdx.x < nb_bounds) bounds_body(t, n_tris, part, blockIdx.x, nb_bounds);
 }
 
 // --------------------------------------------------------------- params
@@ -151,7 +154,7 @@ SG_D void fold_params(GridHeader* h, const BuildArgs& a, const uint32_t (*s_red)
 // GridBuilder::computeResolution (GridBuilder.h:41-62), the two-level top
 // resolution (FastTwoLevelGridBuilder.h:87-102) and the cell sizes of
 // FastSimpleGridBuilder.h:114-117 / SimpleGrid.h:99-107.  One block: first
-// the bounds partials of k_bounds' nb blocks are folded into the header;
+// the bounds partials of the nb bounds blocks are folded into the header;
 // with derive the block then fills the leaf cell size table.
 SG_D void params_body(GridHeader* h, const BuildArgs& a, const uint32_t* __restrict__ part, uint32_t nb, int derive) {
     __shared__ uint32_t s_red[6][kBlock / 64];
@@ -390,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_top_emit(TriSoA t, uint32_t n, GridH
 // src/Control/Algorithm/TwoLevelBuildKernels.h:458-536): a neighbour compare
 // on the sorted keys writes {first, last+1} of every key below C; the dummy
 // key's range is not stored.  Also records P_valid = index of the first
-// dummy key.  Cells were zeroed by k_init.
+// dummy key.  Cells were zeroed by the build's first launch.
 SG_D void ranges_body(const uint32_t* __restrict__ keys, const uint32_t* d_n, uint32_t cap, const uint32_t* d_C,
                       uint32_t* __restrict__ cells, uint32_t* d_pvalid, uint32_t bid, uint32_t nb) {
     const uint32_t n = min(*d_n, cap);
@@ -1192,17 +1195,16 @@ static uint32_t refs_blocks(uint32_t cap) {
     return blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks);
 }
 
-void launch_init(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, hipStream_t st) {
-    hipLaunchKernelGGL(k_init, dim3(init_blocks(segs)), dim3(kBlock), 0, st, h, n_tris, segs);
-}
-
 size_t bounds_blocks(uint32_t n) {
     const size_t b = ((size_t)n + kBlock * 4 - 1) / (kBlock * 4);   // 1024 triangles per block, four per lane
     return b < 1 ? 1 : (b > 4096 ? 4096 : b);
 }
 
-void launch_bounds(const TriSoA& t, uint32_t n, uint32_t* part, hipStream_t st) {
-    hipLaunchKernelGGL(k_bounds, dim3((uint32_t)bounds_blocks(n)), dim3(kBlock), 0, st, t, n, part);
+void launch_init_bounds(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, const TriSoA& t, uint32_t* part,
+                        hipStream_t st) {
+    const uint32_t nb = (uint32_t)bounds_blocks(n_tris);
+    const uint32_t zb = init_blocks(segs);
+    hipLaunchKernelGGL(k_init_bounds, dim3(nb > zb ? nb : zb), dim3(kBlock), 0, st, h, n_tris, segs, t, part, nb);
 }
 
 void launch_params(GridHeader* h, const BuildArgs& a, const uint32_t* part, uint32_t n, bool derive,

@@ -48,7 +48,8 @@ struct RaySoAOut {
     float* a[7];
 };
 
-// byte ranges zeroed by k_init (multiples of 16 bytes, 16-byte aligned)
+// byte ranges zeroed by the first launch of a build (k_init_bounds,
+// k_multi_init; multiples of 16 bytes, 16-byte aligned)
 struct ZeroSegs {
     void* ptr[8];
     size_t bytes[8];
@@ -91,10 +92,11 @@ SG_D void write_ref_record(float* __restrict__ refs, uint32_t i, const float* __
 }
 void launch_ref_records(const uint32_t* indices, const uint32_t* d_valid, const uint32_t* d_gate, uint32_t cap,
                         const float* rec, uint32_t n_tris, float* refs, hipStream_t st);
-void launch_init(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, hipStream_t st);
-// per-block bounds partials (6 words per block, bounds_blocks(n) blocks)
+// zero the segments, reset the header and leave the per-block bounds
+// partials (6 words per block, bounds_blocks(n) blocks), in one launch
 size_t bounds_blocks(uint32_t n);
-void launch_bounds(const TriSoA& t, uint32_t n, uint32_t* part, hipStream_t st);
+void launch_init_bounds(GridHeader* h, uint32_t n_tris, const ZeroSegs& segs, const TriSoA& t, uint32_t* part,
+                        hipStream_t st);
 // fold the partials into the header; with derive, resolution + cell sizes
 void launch_params(GridHeader* h, const BuildArgs& a, const uint32_t* part, uint32_t n, bool derive,
                    hipStream_t st);
@@ -110,7 +112,7 @@ void launch_leaf_cells(GridHeader* h, const BuildArgs& a, const uint32_t* ranges
                        uint32_t* dbg, uint32_t* sums, hipStream_t st);
 // leaf pairs, sorted by key within each leaf tile.  sums
 // (leaf_sums_words(cap_P1)) and info (leaf_info_words(cap_P1)) must be zero
-// on entry (k_init zeroes them).
+// on entry (the build's first launch zeroes them).
 size_t leaf_tiles(uint32_t n);
 size_t leaf_sums_words(uint32_t n);
 size_t leaf_info_words(uint32_t n);
@@ -127,7 +129,8 @@ struct LeafLarge {
 // sort is needed; with `large`, such tiles are emitted in parallel and left
 // to the global sort (launch_leaf_sort_fixup + launch_ref_records).
 // Without `large` the tiles also write the leaf ranges (`leaves`, zeroed by
-// k_init) and the leaf keys only with a.keep_debug; with it, every key is
+// the build's first launch) and the leaf keys only with a.keep_debug; with
+// it, every key is
 // written for the global sort and k_ranges.
 void launch_leaf_emit(const TriSoA& t, GridHeader* h, const BuildArgs& a, const uint32_t* top_keys,
                       const uint32_t* top_vals, const uint32_t* ranges, const uint32_t* cells, uint32_t* keys,
@@ -162,7 +165,7 @@ struct UniJob {
     uint32_t n;               // triangles
     GridHeader* h;
     BuildArgs a;              // as the grid's rebuild (fixed resolution)
-    ZeroSegs z;               // what the grid's k_init zeroes
+    ZeroSegs z;               // what the grid's first build launch zeroes
     uint32_t* part;           // bounds partials
     uint32_t nb_bounds;       // bounds_blocks(n): blocks of the bounds pass
     uint32_t* sums;           // top emission tile sums
