@@ -1302,19 +1302,16 @@ void launch_ref_records(const uint32_t* indices, const uint32_t* d_valid, const 
 // ------------------------------------------------ many uniform grids, one launch
 // k_multi_*: one launch of a phase for every job of an entry table; each
 // block runs the phase's body for its job (see UniJob)
+// zeroing and bounds of every job in one launch, as k_init_bounds
 __global__ __launch_bounds__(kBlock) void k_multi_init(const UniJob* __restrict__ jobs,
                                                        const MultiEntry* __restrict__ e, uint32_t ne) {
     uint32_t bid, nb;
     const UniJob& j = jobs[multi_entry(e, ne, blockIdx.x, gridDim.x, bid, nb)];
     init_body(j.h, j.n, j.z, bid, nb);
-}
-
-__global__ __launch_bounds__(kBlock) void k_multi_bounds(const UniJob* __restrict__ jobs,
-                                                         const MultiEntry* __restrict__ e, uint32_t ne) {
-    uint32_t bid, nb;
-    const UniJob& j = jobs[multi_entry(e, ne, blockIdx.x, gridDim.x, bid, nb)];
-    const TriSoA t = j.t;
-    bounds_body(t, j.n, j.part, bid, nb);
+    if (bid < j.nb_bounds) {
+        const TriSoA t = j.t;
+        bounds_body(t, j.n, j.part, bid, j.nb_bounds);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_multi_params(const UniJob* __restrict__ jobs,
@@ -1360,8 +1357,7 @@ __global__ __launch_bounds__(kBlock) void k_multi_refs(const UniJob* __restrict_
 
 uint32_t multi_blocks(const UniJob& j, MultiPhase ph) {
     switch (ph) {
-        case MP_INIT: return init_blocks(j.z);
-        case MP_BOUNDS: return j.nb_bounds;
+        case MP_INIT: return init_blocks(j.z) > j.nb_bounds ? init_blocks(j.z) : j.nb_bounds;
         case MP_PARAMS: return 1;
         case MP_TOP_COUNT:
         case MP_TOP_EMIT: return (j.n + kTile - 1) / kTile;
@@ -1377,7 +1373,6 @@ void launch_multi(MultiPhase ph, const UniJob* jobs, const MultiEntry* e, uint32
     const dim3 g(blocks), b(kBlock);
     switch (ph) {
         case MP_INIT: hipLaunchKernelGGL(k_multi_init, g, b, 0, st, jobs, e, ne); break;
-        case MP_BOUNDS: hipLaunchKernelGGL(k_multi_bounds, g, b, 0, st, jobs, e, ne); break;
         case MP_PARAMS: hipLaunchKernelGGL(k_multi_params, g, b, 0, st, jobs, e, ne); break;
         case MP_TOP_COUNT: hipLaunchKernelGGL(k_multi_top_count, g, b, 0, st, jobs, e, ne); break;
         case MP_TOP_EMIT: hipLaunchKernelGGL(k_multi_top_emit, g, b, 0, st, jobs, e, ne); break;

@@ -182,7 +182,7 @@ struct UniJob {
 struct MultiEntry {
     uint32_t job, b0;
 };
-enum MultiPhase { MP_INIT, MP_BOUNDS, MP_PARAMS, MP_TOP_COUNT, MP_TOP_EMIT, MP_RANGES, MP_REFS, MP_COUNT };
+enum MultiPhase { MP_INIT, MP_PARAMS, MP_TOP_COUNT, MP_TOP_EMIT, MP_RANGES, MP_REFS, MP_COUNT };
 // the job of block b of a launch over `total` blocks (the last entry with
 // b0 <= b), with the block's index and the block count within the job
 SG_D uint32_t multi_entry(const MultiEntry* __restrict__ e, uint32_t ne, uint32_t b, uint32_t total, uint32_t& bid,

@@ -43,7 +43,7 @@ struct PlanLaunch {
     uint32_t e0, ne, blocks;
 };
 
-// entries of one job in the plan at most: the seven phases plus three
+// entries of one job in the plan at most: the six phases plus three
 // launches per sort pass (keys of at most 32 bits: 8 passes of 4 bits)
 constexpr uint32_t kPlanEntriesPerJob = MP_COUNT + 3 * 8;
 
@@ -75,7 +75,7 @@ struct sg_batch {
 
 // Lays out the rebuild plan of the batch's grids in the pinned staging
 // buffers; returns the number of entries.  The launch order is the one of
-// a single uniform rebuild (init, bounds, params, top count / emit, the
+// a single uniform rebuild (zeroing with bounds, params, top count / emit, the
 // sort passes, ranges, reference records); the sort passes go by radix
 // width, each pass over the grids that sort with that width and have that
 // many passes.
@@ -110,7 +110,7 @@ static uint32_t plan_rebuild(sg_batch* b) {
         L.ne = ne - L.e0;
         if (L.ne) b->plan.push_back(L);
     };
-    for (int ph : {MP_INIT, MP_BOUNDS, MP_PARAMS, MP_TOP_COUNT, MP_TOP_EMIT})
+    for (int ph : {MP_INIT, MP_PARAMS, MP_TOP_COUNT, MP_TOP_EMIT})
         add(false, ph, 0, 0, [&](const UniJob& j) { return multi_blocks(j, (MultiPhase)ph); });
     for (int rb : {8, 6, 4}) {
         int passes = 0;   // at most 8: sort_passes of 32-bit keys at 4 bits
