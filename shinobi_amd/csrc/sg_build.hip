@@ -1177,9 +1177,10 @@ static uint32_t ranges_blocks(uint32_t cap) {
     return blocks < 1 ? 1 : (blocks > 4096 ? 4096 : blocks);
 }
 
+// one reference per thread (each record is a dependent gather)
 static uint32_t refs_blocks(uint32_t cap) {
-    const uint32_t blocks = (cap + kBlock * 4 - 1) / (kBlock * 4);
-    return blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks);
+    const uint32_t blocks = (cap + kBlock - 1) / kBlock;
+    return blocks < 1 ? 1 : (blocks > 32768 ? 32768 : blocks);
 }
 
 size_t bounds_blocks(uint32_t n) {
