@@ -515,7 +515,7 @@ __global__ __launch_bounds__(kBlock) void k_leaf_cells_write(GridHeader* h, Buil
 // its pairs in emission order and raises GridHeader::leaf_sort, which enables
 // a global stable radix sort of the accepted pairs (a no-op otherwise).
 constexpr uint32_t kLeafTile = 512;                   // nominal sorted top pairs per leaf tile
-constexpr int kLeafBlock = 512;                       // threads of the leaf count / emit kernels
+constexpr int kLeafBlock = 512;                       // threads of the leaf emission kernel
 constexpr uint32_t kLeafStage = 4096;                 // leaf pairs a tile sorts in LDS
 constexpr int kPayBits = 12;                          // payload: top pair index within the tile
 constexpr int kLocalBits = 16;                        // tile-local key bits the LDS sort handles
