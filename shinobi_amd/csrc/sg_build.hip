@@ -674,7 +674,19 @@ __global__ __launch_bounds__(kBlock) void k_leaf_count(TriSoA t, GridHeader* h,
         const uint32_t c = leaf_pair(t, L, h, key, tri, cells, st);
         if (dbg_counts) dbg_counts[i] = c;
         uint32_t acc = 0;
-        if (c) leaf_visit(L, st, [&](uint32_t, bool ok) { acc += ok ? 1u : 0u; });
+        if (c) {
+            // a box of one leaf cell on some axis has no plane test: every
+            // cell is accepted unless its key reaches C2, and the largest
+            // key is that of the far corner -- then the count is the volume
+            const bool plane = st.hp[0] - st.lo[0] > 1 && st.hp[1] - st.lo[1] > 1 && st.hp[2] - st.lo[2] > 1;
+            const uint32_t crx = f2u(st.cres.x), crxy = f2u(st.cres.x * st.cres.y);
+            const uint32_t kmax = (uint32_t)(st.hp[0] - 1) + (uint32_t)(st.hp[1] - 1) * crx +
+                                  (uint32_t)(st.hp[2] - 1) * crxy + st.px;
+            if (!plane && kmax < L.C2)
+                acc = c;
+            else
+                leaf_visit(L, st, [&](uint32_t, bool ok) { acc += ok ? 1u : 0u; });
+        }
         if (pacc) {   // per-pair totals for the parallel emission of oversize tiles
             pacc[i] = acc;
             prej[i] = c - acc;
