@@ -371,8 +371,8 @@ sg_status sg_stream_wait_event(sg_ctx *ctx, void *stream, void *ev);
 typedef struct sg_batch sg_batch;
 sg_status sg_batch_create(sg_ctx *ctx, uint32_t max_items, sg_batch **out);
 sg_status sg_batch_destroy(sg_batch *batch);
-/* drop every item and the rebuild plan (commit again before the next
- * launch or rebuild) */
+/* drop every item, the uploaded table and the rebuild plan: until the next
+ * sg_batch_commit, sg_batch_launch and sg_batch_rebuild do nothing */
 sg_status sg_batch_clear(sg_batch *batch);
 /* append one item: device SoA rays, n rays, device outputs (prim, t).  The
  * batch keeps the pointers: the grid, rays and outputs must outlive the

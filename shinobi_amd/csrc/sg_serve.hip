@@ -193,7 +193,10 @@ sg_status sg_batch_clear(sg_batch* b) {
     b->items.clear();
     b->grids.clear();
     b->next_chunk = 0;
-    // the rebuild plan goes with the items (its grids may be destroyed next)
+    // the uploaded table and the rebuild plan go with the items (their grids
+    // may be destroyed next): a launch or rebuild before the next commit
+    // does nothing
+    b->dev_split = ItemSplit{};
     b->plan.clear();
     b->plan_uniform.clear();
     b->plan_two.clear();

@@ -312,6 +312,11 @@ def test_rebuild_many_graph_matches_oracle(ctx, oracle):
         batch.rebuild(branches=4)
     batch.clear()
     batch.rebuild(branches=4)
+    # a cleared batch launches nothing: the outputs keep what they hold
+    outs[0][0].upload(np.full(rb.n, 5, np.uint32))
+    batch.launch()
+    ctx.sync()
+    assert (outs[0][0].download(np.uint32, rb.n) == 5).all()
     for it, (p, t) in zip(items, outs):
         batch.add(it[3], rb, p, t)
     batch.commit()
